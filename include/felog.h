@@ -1,0 +1,213 @@
+/*
+ * felog.h - C ABI of libfelog_hip.so, the MI355X walk -> SGNS hot path of
+ * FeLoG (information-centric graph embedding).
+ *
+ * One context owns one GPU's copy of the graph (CSR + alias tables), the two
+ * embedding tables, the per-vertex convergence flags and the last round's walk
+ * corpus.  All device memory is owned by the context; every pointer argument
+ * below is a HOST pointer that is borrowed for the duration of the call only.
+ *
+ * Conventions
+ *   - every entry point returns felog_status (0 = OK, < 0 = error class);
+ *     felog_last_error() returns a message for the last failure on a context.
+ *     Nothing in the library calls exit() or assert().
+ *   - calls on one context must be serialised by the caller (one owning host
+ *     thread); distinct contexts are independent.
+ *   - vertex ids are u32, edge counts u64.  The directed edge count (after
+ *     undirecting) must stay below 2^32 in this build.
+ *
+ * Reference interfaces this boundary replaces (paths relative to the
+ * reference repository):
+ *   - felog argv surface: include/option_helper.hpp:94-188 and
+ *     src/word2vec.cu:2269-2286 (mirrored field by field in felog_options)
+ *   - GraphEngine::load_graph: include/graph.hpp:712-919
+ *   - WalkEngine::random_walk: include/walk.hpp:529-542, 630-840, 848-1122
+ *   - train_corpus_cuda: src/word2vec.cu:2193 (declared at
+ *     src/examples/felog.cpp:26), TrainModel src/word2vec.cu:1922-2180
+ */
+#ifndef FELOG_H
+#define FELOG_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t felog_status;
+
+enum {
+    FELOG_OK = 0,
+    FELOG_ERR_ARG = -1,        /* bad argument or option value            */
+    FELOG_ERR_IO = -2,         /* file missing, short or malformed         */
+    FELOG_ERR_HIP = -3,        /* a HIP runtime call failed                */
+    FELOG_ERR_OOM = -4,        /* device allocation failed                 */
+    FELOG_ERR_INVARIANT = -5,  /* an internal consistency check failed     */
+    FELOG_ERR_STATE = -6,      /* call made in the wrong order             */
+    FELOG_ERR_CAPACITY = -7    /* a bounded device buffer overflowed       */
+};
+
+/* Precision of the embedding tables. */
+enum { FELOG_FP32 = 0, FELOG_BF16 = 1 };
+
+/*
+ * Parsed felog options.  felog_default_options() fills the reference
+ * defaults; zero-valued fields that the reference treats as "use default"
+ * keep that meaning.
+ */
+typedef struct felog_options {
+    /* graph / walk flags (include/option_helper.hpp:94-188) */
+    uint32_t v_num;            /* -v                                         */
+    uint32_t walkers;          /* -w  (0 -> v_num; one walker per vertex)    */
+    uint32_t min_L;            /* --min_L (0 -> 20, include/walk.hpp:511)    */
+    uint32_t min_R;            /* --min_R (0 -> 5,  include/walk.hpp:506)    */
+    uint32_t eval_batch;       /* --batch_size (16384): evaluation chunk     */
+    int32_t make_undirected;   /* --make-undirected                          */
+    /* word2vec flags (src/word2vec.cu:2269-2286) */
+    uint32_t size;             /* -size   (100)                              */
+    uint32_t window;           /* -window (5)                                */
+    uint32_t negative;         /* -negative (5)                              */
+    uint32_t batch_size;       /* -batch-size: centres sharing one negative
+                                  set (0 -> 2*window+1, include/dsgl.hpp:832) */
+    float sample;              /* -sample (1e-3)                             */
+    float alpha;               /* -alpha  (0.025)                            */
+    uint32_t min_count;        /* -min-count (5)                             */
+    int32_t binary;            /* -binary (0)                                */
+    /* build switches */
+    int32_t precision;         /* FELOG_FP32 or FELOG_BF16                   */
+    uint32_t max_walk_len;     /* hard walk cap in vertices (1000)           */
+    uint64_t overflow_capacity;/* vertices of walk storage beyond the fixed
+                                  per-walker slot (0 -> 1<<24)               */
+    uint64_t seed;             /* Philox key                                 */
+    int32_t compat_lr;         /* 1: ExponentialDecayLR(0.025,0.1) ignores
+                                  -alpha (src/word2vec.cu:1958,2001)         */
+    int32_t compat_vocab_break;/* 1: vocab index 0 and min-count-dropped
+                                  vertices end a sentence
+                                  (src/word2vec.cu:1202-1205)                */
+    uint32_t max_rounds;       /* felog_run() round limit (0 -> no limit)    */
+    int32_t device;            /* HIP device ordinal                         */
+} felog_options;
+
+typedef struct felog_ctx felog_ctx;
+
+/* Summary of the corpus produced by the last felog_walk_round(); the token
+ * data stays on the device inside the context. */
+typedef struct felog_corpus {
+    uint32_t round;            /* 1-based round number                       */
+    uint32_t max_len;          /* longest walk in vertices                   */
+    uint64_t n_walks;          /* non-empty walks (walkers started)          */
+    uint64_t n_tokens;         /* sum of walk lengths                        */
+    uint64_t n_steps;          /* sum of (length - 1): vertices appended     */
+} felog_corpus;
+
+typedef struct felog_stats {
+    uint64_t v_num, e_num;           /* vertices, directed CSR entries      */
+    uint64_t vocab_size;             /* vertices with degree >= min_count   */
+    uint32_t rounds_done;
+    uint32_t n_unconverged;          /* after the last evaluation           */
+    uint64_t walk_steps;             /* last round                          */
+    uint64_t walk_tokens;            /* last round                          */
+    uint64_t train_words;            /* last round: tokens read pre-subsample */
+    uint64_t train_kept;             /* last round: tokens kept              */
+    uint64_t train_pairs;            /* last round: (centre, context) pairs */
+    uint64_t train_groups;           /* last round: SGNS groups             */
+    uint64_t overflow_used;          /* last round: overflow vertices used  */
+    double build_ms;                 /* CSR + alias + vocab + init          */
+    double walk_kernel_ms;           /* last round, HIP events, walk kernel */
+    double compact_kernel_ms;        /* last round, corpus compaction       */
+    double train_kernel_ms;          /* last round, HIP events, SGNS kernel */
+    double eval_kernel_ms;           /* last evaluation kernel              */
+    double walk_ms_total, train_ms_total, eval_ms_total;  /* all rounds     */
+} felog_stats;
+
+/* ---- lifetime ---------------------------------------------------------- */
+void felog_default_options(felog_options *opt);
+felog_status felog_create(felog_ctx **out, const felog_options *opt);
+felog_status felog_destroy(felog_ctx *ctx);
+const char *felog_last_error(const felog_ctx *ctx);
+
+/* ---- graph input (include/storage.hpp:40-61, include/graph.hpp:741-772) */
+/* Binary edge file: packed little-endian {u32 src, u32 dst, f32 data}
+ * records, no header.  make_undirected appends each edge's reverse right
+ * after it. */
+felog_status felog_load_edges(felog_ctx *ctx, const char *bin_path,
+                              uint32_t v_num, int make_undirected);
+felog_status felog_set_edges(felog_ctx *ctx, const uint32_t *src,
+                             const uint32_t *dst, const float *w, uint64_t m,
+                             uint32_t v_num, int make_undirected);
+
+/* CSR (neighbour order = input order, include/graph.hpp:531-575), alias
+ * tables, vocabulary, table initialisation (src/word2vec.cu:864-902),
+ * unigram sampler and subsampling thresholds. */
+felog_status felog_build(felog_ctx *ctx);
+
+/* ---- the hot path ------------------------------------------------------ */
+/* One round of walks: one walker per unconverged vertex
+ * (include/walk.hpp:547-610, 848-1122). */
+felog_status felog_walk_round(felog_ctx *ctx, uint32_t round,
+                              felog_corpus *out);
+/* Skip-gram negative-sampling training over the last walk corpus
+ * (replaces src/word2vec.cu:1133-1273 and skip_gram_kernel :442-536). */
+felog_status felog_train_round(felog_ctx *ctx, const felog_corpus *corpus);
+/* Train over a caller-supplied corpus (the SyncQueue hand-off of
+ * src/word2vec.cu:1990-1998): sentence s is tokens[offsets[s]:offsets[s+1]]. */
+felog_status felog_train_corpus(felog_ctx *ctx, const uint64_t *offsets,
+                                const uint32_t *tokens, uint64_t n_sentences,
+                                uint32_t round);
+/* Convergence evaluation (src/word2vec.cu:2016-2080): flags vertices whose
+ * mean cosine to <= 30 neighbours exceeds 0.65; *stop = 1 when the
+ * unconverged count stopped shrinking (ratio > 0.7). */
+felog_status felog_evaluate(felog_ctx *ctx, uint32_t *n_unconverged,
+                            int *stop);
+/* Rounds until the stop rule fires or max_rounds is reached. */
+felog_status felog_run(felog_ctx *ctx);
+
+/* ---- read-back (caller-allocated host buffers) ------------------------- */
+felog_status felog_corpus_copy_out(felog_ctx *ctx, uint64_t *offsets,
+                                   uint32_t *tokens, uint32_t *walker_ids);
+felog_status felog_csr_copy_out(felog_ctx *ctx, uint64_t *row_ptr,
+                                uint32_t *col, float *w);
+felog_status felog_alias_copy_out(felog_ctx *ctx, uint32_t *thresh,
+                                  uint32_t *alias_idx);
+felog_status felog_vocab_copy_out(felog_ctx *ctx, uint32_t *vocab_order,
+                                  uint32_t *degree);
+felog_status felog_unigram_copy_out(felog_ctx *ctx, uint32_t *thresh,
+                                    uint32_t *alias_idx);
+felog_status felog_flags_copy_out(felog_ctx *ctx, uint8_t *flags,
+                                  float *scores);
+/* Rows indexed by vertex id, widened to f32; either pointer may be NULL. */
+felog_status felog_embeddings_copy_out(felog_ctx *ctx, float *syn0,
+                                       float *syn1neg);
+felog_status felog_embeddings_copy_in(felog_ctx *ctx, const float *syn0,
+                                      const float *syn1neg);
+
+/* ---- writers (include/path.hpp:87-97, src/word2vec.cu:2117-2127) ------- */
+felog_status felog_write_walks(felog_ctx *ctx, const char *prefix);
+felog_status felog_write_embeddings(felog_ctx *ctx, const char *path,
+                                    int binary);
+
+felog_status felog_stats_get(felog_ctx *ctx, felog_stats *out);
+
+/* ---- test hook --------------------------------------------------------- */
+/* Runs the SGNS kernel on a caller-supplied corpus with updates diverted:
+ * instead of touching the tables it writes, for every group, the f32 logits
+ * and the table deltas into caller buffers.  Used to compare the kernel's
+ * arithmetic with an FP64 restatement on race-free inputs.
+ *   groups_out: per group {sentence, first centre, n_centres, n_in, n_out}
+ *   rows_out:   per group 64 input + 32 output vertex ids (0xFFFFFFFF = pad)
+ *   logits_out: per group 64 x 32 f32
+ *   dv_out:     per group 64 x size f32 (input-row deltas)
+ *   du_out:     per group 32 x size f32 (output-row deltas)
+ * max_groups bounds all buffers; *n_groups receives the count. */
+felog_status felog_debug_sgns(felog_ctx *ctx, const uint64_t *offsets,
+                              const uint32_t *tokens, uint64_t n_sentences,
+                              uint32_t round, float alpha,
+                              uint32_t max_groups, uint32_t *n_groups,
+                              uint32_t *groups_out, uint32_t *rows_out,
+                              float *logits_out, float *dv_out, float *du_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FELOG_H */

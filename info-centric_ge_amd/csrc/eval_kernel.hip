@@ -1,0 +1,134 @@
+// eval_kernel.hip - convergence evaluation of the walk rounds.
+//
+// Restates src/word2vec.cu:2016-2080 with the pair kernel of :1448-1506:
+// every vocabulary vertex that is not yet converged is scored by the mean
+// cosine similarity (f32) of its syn0 row to at most 30 neighbours
+// (EVALUATION_NEIGHBOUR_NUM, :45) - all of them when the degree is <= 30,
+// otherwise 30 distinct neighbours drawn with Floyd's algorithm from Philox
+// (the reference shuffles with a std::random_device seed, :1787-1792).
+// Scores above 0.65 (:46) set the vertex's flag, after which no walker
+// starts there.
+#include <hip/hip_runtime.h>
+#include "felog_dev.h"
+#include "felog_internal.h"
+
+namespace felog {
+
+namespace {
+
+constexpr uint32_t kEvalNbrs = 30;
+constexpr float kConverge = 0.65f;
+
+template <typename T>
+__device__ __forceinline__ float ld(const T *p);
+template <>
+__device__ __forceinline__ float ld<float>(const float *p) { return *p; }
+template <>
+__device__ __forceinline__ float ld<uint16_t>(const uint16_t *p) { return bf16_to_f32(*p); }
+
+__device__ __forceinline__ float wave_sum(float x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_ptr,
+                                              const uint32_t *__restrict__ col,
+                                              const uint8_t *__restrict__ in_vocab,
+                                              uint8_t *__restrict__ flags, float *__restrict__ scores,
+                                              const T *__restrict__ syn0, uint32_t V, uint32_t dp,
+                                              uint32_t d, uint32_t round, uint64_t seed,
+                                              unsigned long long *counters) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    unsigned long long n_eval = 0, n_conv = 0;
+    for (uint32_t v = wave; v < V; v += nwaves) {
+        if (!in_vocab[v] || flags[v]) continue;
+        const uint64_t b = row_ptr[v];
+        const uint32_t deg = (uint32_t)(row_ptr[v + 1] - b);
+        float score = 0.f;
+        if (deg > 0) {
+            const uint32_t cnt = deg < kEvalNbrs ? deg : kEvalNbrs;
+            // chosen neighbour index held by lane i (i < cnt)
+            uint32_t mine = lane;
+            if (deg > kEvalNbrs) {
+                mine = 0xFFFFFFFFu;
+                for (uint32_t i = 0; i < kEvalNbrs; ++i) {
+                    const uint32_t j = deg - kEvalNbrs + i;
+                    const uint32_t t = draw(seed, v, DOM_EVAL | i, 0xFFFFFFFFu, round).x % (j + 1);
+                    const bool dup = __ballot(lane < i && mine == t) != 0;
+                    if (lane == i) mine = dup ? j : t;
+                }
+            }
+            float nn = 0.f;
+            for (uint32_t e = lane; e < d; e += 64) {
+                const float x = ld(syn0 + (uint64_t)v * dp + e);
+                nn += x * x;
+            }
+            nn = wave_sum(nn);
+            const float nv = sqrtf(nn);
+            float sum = 0.f;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const uint32_t idx = __shfl(mine, (int)i);
+                const uint32_t u = col[b + idx];
+                float dot = 0.f, mm = 0.f;
+                for (uint32_t e = lane; e < d; e += 64) {
+                    const float x = ld(syn0 + (uint64_t)v * dp + e);
+                    const float y = ld(syn0 + (uint64_t)u * dp + e);
+                    dot += x * y;
+                    mm += y * y;
+                }
+                dot = wave_sum(dot);
+                mm = wave_sum(mm);
+                const float nu = sqrtf(mm);
+                sum += (nv == 0.f || nu == 0.f) ? 0.f : dot / (nv * nu);
+            }
+            score = sum / (float)cnt;
+        }
+        if (lane == 0) {
+            scores[v] = score;
+            if (score > kConverge) {
+                flags[v] = 1;
+                ++n_conv;
+            } else {
+                ++n_eval;
+            }
+        }
+    }
+    if (lane == 0) {
+        if (n_eval) atomicAdd(&counters[C_EVALNUM], n_eval);
+        if (n_conv) atomicAdd(&counters[C_EVALCONV], n_conv);
+    }
+}
+
+}  // namespace
+
+#define HIP_TRY(x)                              \
+    do {                                        \
+        hipError_t e_ = (x);                    \
+        if (e_ != hipSuccess) return e_;        \
+    } while (0)
+
+hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms) {
+    HIP_TRY(hipMemsetAsync(c->d_counters + C_EVALNUM, 0, 16, c->stream));
+    const unsigned blocks = c->cu_count * 8;
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (c->opt.precision == FELOG_BF16)
+        k_eval<uint16_t><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
+                                                       c->d_scores, (const uint16_t *)c->d_syn0, c->V, c->dp,
+                                                       c->d, round, c->opt.seed,
+                                                       (unsigned long long *)c->d_counters);
+    else
+        k_eval<float><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
+                                                    c->d_scores, (const float *)c->d_syn0, c->V, c->dp, c->d,
+                                                    round, c->opt.seed, (unsigned long long *)c->d_counters);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));
+    HIP_TRY(hipMemcpy(c->h_counters, c->d_counters, kNumCounters * 8, hipMemcpyDeviceToHost));
+    return hipSuccess;
+}
+
+}  // namespace felog

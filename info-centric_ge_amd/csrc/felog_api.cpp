@@ -1,0 +1,605 @@
+// felog_api.cpp - the C ABI of libfelog_hip.so (declared in include/felog.h).
+//
+// Host orchestration of one FeLoG run on one GPU: graph input, device build
+// (CSR, alias tables, vocabulary, tables), walk rounds, SGNS training,
+// convergence evaluation, read-back and the reference's file formats.
+// The round structure follows TrainModel (src/word2vec.cu:1922-2180) and
+// WalkEngine::internal_random_walk (include/walk.hpp:630-840), made serial
+// and deterministic: round r walks every unconverged vertex, trains on that
+// corpus with alpha_r, and from round min_R on evaluates and flags vertices.
+#include <hip/hip_runtime.h>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "felog_internal.h"
+
+using namespace felog;
+
+namespace {
+
+felog_status hip_fail(felog_ctx *c, hipError_t e, const char *what) {
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? FELOG_ERR_OOM : FELOG_ERR_HIP;
+}
+
+felog_status fail(felog_ctx *c, felog_status code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define TRY(c, x)                                               \
+    do {                                                        \
+        hipError_t e_ = (x);                                    \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #x);     \
+    } while (0)
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Learning rate of round r (1-based): ExponentialDecayLR(base, 0.1) with one
+// get_lr() call per round (include/lr_scheduler.hpp:23-31,
+// src/word2vec.cu:1958,2001).  The reference always uses base 0.025; the
+// default here honours -alpha as the base.
+float round_alpha(const felog_options &o, uint32_t round) {
+    const float base = o.compat_lr ? 0.025f : o.alpha;
+    const float decay = 0.1f;
+    return base * std::exp(-decay * (float)(round - 1));
+}
+
+void free_all(felog_ctx *c) {
+    void *ptrs[] = {c->d_esrc, c->d_edst, c->d_ew, c->d_row_ptr, c->d_col, c->d_w, c->d_vrec, c->d_erec,
+                    c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni_thr,
+                    c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_syn1, c->d_flags,
+                    c->d_scores, c->d_active, c->d_walk_ok, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
+                    c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tmp};
+    for (void *p : ptrs) dfree(p);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+felog_status need_built(felog_ctx *c) {
+    if (!c) return FELOG_ERR_ARG;
+    if (!c->built) return fail(c, FELOG_ERR_STATE, "felog_build has not run");
+    TRY(c, hipSetDevice(c->device));
+    return FELOG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void felog_default_options(felog_options *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->min_L = 20;
+    o->min_R = 5;
+    o->eval_batch = 16384;
+    o->size = 100;
+    o->window = 5;
+    o->negative = 5;
+    o->batch_size = 0;
+    o->sample = 1e-3f;
+    o->alpha = 0.025f;
+    o->min_count = 5;
+    o->precision = FELOG_FP32;
+    o->max_walk_len = 1000;
+    o->overflow_capacity = 1ull << 24;
+    o->seed = 0x5EEDF00DULL;
+}
+
+felog_status felog_create(felog_ctx **out, const felog_options *opt) {
+    if (!out || !opt) return FELOG_ERR_ARG;
+    *out = nullptr;
+    felog_ctx *c = new felog_ctx();
+    c->opt = *opt;
+    felog_options &o = c->opt;
+    if (o.min_L == 0) o.min_L = 20;   // include/walk.hpp:511-514
+    if (o.min_R == 0) o.min_R = 5;    // include/walk.hpp:506-509
+    if (o.max_walk_len == 0) o.max_walk_len = 1000;
+    if (o.overflow_capacity == 0) o.overflow_capacity = 1ull << 24;
+    std::string bad;
+    if (o.size == 0 || o.size > 512) bad = "-size must be in [1, 512]";
+    else if (o.window == 0 || o.window > 26) bad = "-window must be in [1, 26]";
+    else if (o.negative > 24) bad = "-negative must be <= 24";
+    else if (o.precision != FELOG_FP32 && o.precision != FELOG_BF16) bad = "precision must be FP32 or BF16";
+    else if (o.max_walk_len < 2 || o.max_walk_len > (1u << 20)) bad = "max_walk_len must be in [2, 2^20]";
+    else if (!(o.alpha > 0.f)) bad = "-alpha must be > 0";
+    if (bad.empty() && sgns_group_centres(o.batch_size, o.window, o.negative) < 1)
+        bad = "-window/-negative leave no room for a centre in an SGNS group";
+    if (!bad.empty()) {
+        delete c;
+        return FELOG_ERR_ARG;
+    }
+    c->d = o.size;
+    c->dp = (o.size + 31) / 32 * 32;
+    if (o.precision == FELOG_FP32 && c->dp > 320) {
+        delete c;
+        return FELOG_ERR_ARG;  // f32 rows beyond 320 do not fit the SGNS LDS tile
+    }
+    c->device = o.device;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipMalloc(&c->d_counters, kNumCounters * 8);
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_counters, kNumCounters * 8, hipHostMallocDefault);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    if (e != hipSuccess) {
+        free_all(c);
+        delete c;
+        return e == hipErrorOutOfMemory ? FELOG_ERR_OOM : FELOG_ERR_HIP;
+    }
+    c->cu_count = (uint32_t)cus;
+    *out = c;
+    return FELOG_OK;
+}
+
+felog_status felog_destroy(felog_ctx *c) {
+    if (!c) return FELOG_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return FELOG_OK;
+}
+
+const char *felog_last_error(const felog_ctx *c) {
+    return c ? c->err.c_str() : "null context";
+}
+
+felog_status felog_set_edges(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w, uint64_t m,
+                             uint32_t v_num, int make_undirected) {
+    if (!c) return FELOG_ERR_ARG;
+    if (c->have_edges) return fail(c, FELOG_ERR_STATE, "edges already set on this context");
+    if (v_num == 0) return fail(c, FELOG_ERR_ARG, "v_num must be > 0");
+    if (m > 0 && (!src || !dst || !w)) return fail(c, FELOG_ERR_ARG, "null edge array");
+    const uint64_t E = make_undirected ? 2 * m : m;
+    if (E >= (1ull << 32)) return fail(c, FELOG_ERR_ARG, "directed edge count must be < 2^32");
+    TRY(c, hipSetDevice(c->device));
+    c->V = v_num;
+    if (m == 0) {
+        c->E = 0;
+    } else {
+        TRY(c, launch_undirect(c, src, dst, w, m, make_undirected != 0));
+        uint64_t bad = 0;
+        TRY(c, hipMemcpy(&bad, c->d_counters + C_BADEDGE, 8, hipMemcpyDeviceToHost));
+        if (bad) {
+            dfree(c->d_esrc); dfree(c->d_edst); dfree(c->d_ew);
+            c->d_esrc = c->d_edst = nullptr; c->d_ew = nullptr;
+            return fail(c, FELOG_ERR_ARG, std::to_string(bad) + " edges have a vertex id >= v_num");
+        }
+    }
+    c->have_edges = true;
+    return FELOG_OK;
+}
+
+felog_status felog_load_edges(felog_ctx *c, const char *path, uint32_t v_num, int make_undirected) {
+    if (!c || !path) return FELOG_ERR_ARG;
+    const int fd = open(path, O_RDONLY);
+    if (fd < 0) return fail(c, FELOG_ERR_IO, std::string("cannot open ") + path);
+    struct stat sb;
+    if (fstat(fd, &sb) != 0) {
+        close(fd);
+        return fail(c, FELOG_ERR_IO, std::string("cannot stat ") + path);
+    }
+    const uint64_t bytes = (uint64_t)sb.st_size;
+    if (bytes % 12 != 0) {
+        close(fd);
+        return fail(c, FELOG_ERR_IO, std::string(path) + ": size is not a multiple of 12-byte edge records");
+    }
+    const uint64_t m = bytes / 12;
+    std::vector<uint32_t> src(m), dst(m);
+    std::vector<float> w(m);
+    if (m > 0) {
+        void *map = mmap(nullptr, bytes, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (map == MAP_FAILED) {
+            close(fd);
+            return fail(c, FELOG_ERR_IO, std::string("cannot map ") + path);
+        }
+        const unsigned char *b = (const unsigned char *)map;
+        for (uint64_t i = 0; i < m; ++i) {
+            std::memcpy(&src[i], b + 12 * i, 4);
+            std::memcpy(&dst[i], b + 12 * i + 4, 4);
+            std::memcpy(&w[i], b + 12 * i + 8, 4);
+        }
+        munmap(map, bytes);
+    }
+    close(fd);
+    return felog_set_edges(c, src.data(), dst.data(), w.data(), m, v_num, make_undirected);
+}
+
+felog_status felog_build(felog_ctx *c) {
+    if (!c) return FELOG_ERR_ARG;
+    if (!c->have_edges) return fail(c, FELOG_ERR_STATE, "no edges: call felog_set_edges or felog_load_edges");
+    if (c->built) return fail(c, FELOG_ERR_STATE, "already built");
+    TRY(c, hipSetDevice(c->device));
+    const double t0 = now_ms();
+    if (c->E == 0) {
+        // a graph without edges still gets its per-vertex arrays
+        TRY(c, hipMalloc(&c->d_esrc, 4));
+        TRY(c, hipMalloc(&c->d_edst, 4));
+        TRY(c, hipMalloc(&c->d_ew, 4));
+    }
+    TRY(c, build_csr(c));
+    std::vector<uint32_t> deg;
+    TRY(c, build_vocab_and_tables(c, deg));
+    TRY(c, build_alias(c));
+    TRY(c, init_tables(c));
+    const uint32_t V = c->V;
+    TRY(c, hipMalloc(&c->d_flags, V));
+    TRY(c, hipMemset(c->d_flags, 0, V));
+    TRY(c, hipMalloc(&c->d_scores, (uint64_t)V * 4));
+    TRY(c, hipMemset(c->d_scores, 0, (uint64_t)V * 4));
+    // walk buffers
+    c->walkers = c->opt.walkers ? c->opt.walkers : V;
+    uint32_t slot = (c->opt.min_L + 2 + 3) / 4 * 4;
+    if (slot > c->opt.max_walk_len) slot = c->opt.max_walk_len;
+    c->slot = slot;
+    const uint64_t W = c->walkers;
+    TRY(c, hipMalloc(&c->d_active, W * 4));
+    TRY(c, hipMalloc(&c->d_walk_ok, W));
+    TRY(c, hipMalloc(&c->d_slots, W * slot * 4));
+    TRY(c, hipMalloc(&c->d_lens, W * 4));
+    TRY(c, hipMalloc(&c->d_ovf_base, W * 8));
+    c->ovf_cap = c->opt.max_walk_len > slot ? c->opt.overflow_capacity : 0;
+    TRY(c, hipMalloc(&c->d_ovf, (c->ovf_cap ? c->ovf_cap : 1) * 4));
+    c->last_eva_num = (uint32_t)c->vocab_size;
+    c->n_unconverged = (uint32_t)c->vocab_size;
+    c->st.build_ms = now_ms() - t0;
+    c->st.v_num = V;
+    c->st.e_num = c->E;
+    c->st.vocab_size = c->vocab_size;
+    c->built = true;
+    return FELOG_OK;
+}
+
+felog_status felog_walk_round(felog_ctx *c, uint32_t round, felog_corpus *out) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (round == 0) return fail(c, FELOG_ERR_ARG, "rounds are numbered from 1");
+    float wms = 0.f, cms = 0.f;
+    c->corpus = Corpus();
+    TRY(c, run_walks(c, round, 0, &wms));
+    const uint64_t fail_n = c->h_counters[C_OVF_FAIL];
+    const uint64_t steps = c->h_counters[C_STEPS];
+    const uint64_t maxlen = c->h_counters[C_MAXLEN];
+    c->st.overflow_used = c->h_counters[C_OVF_NEXT];
+    TRY(c, compact_corpus(c, (uint32_t)c->corpus.n_walks, &cms));
+    c->corpus.round = round;
+    c->corpus.n_steps = steps;
+    c->corpus.max_len = (uint32_t)maxlen;
+    c->corpus.valid = true;
+    c->corpus.from_walk = true;
+    c->st.walk_steps = steps;
+    c->st.walk_tokens = c->corpus.n_tokens;
+    c->st.walk_kernel_ms = wms;
+    c->st.compact_kernel_ms = cms;
+    c->st.walk_ms_total += wms + cms;
+    if (out) {
+        out->round = round;
+        out->max_len = c->corpus.max_len;
+        out->n_walks = c->corpus.n_walks;
+        out->n_tokens = c->corpus.n_tokens;
+        out->n_steps = steps;
+    }
+    if (fail_n)
+        return fail(c, FELOG_ERR_CAPACITY,
+                    std::to_string(fail_n) + " walks outgrew the overflow pool; raise overflow_capacity");
+    return FELOG_OK;
+}
+
+static felog_status train_device_corpus(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n,
+                                        uint32_t round, float alpha) {
+    float ms = 0.f;
+    TRY(c, run_sgns(c, d_off, d_tok, n, round, alpha, nullptr, &ms));
+    c->st.train_words = c->h_counters[C_WORDS];
+    c->st.train_kept = c->h_counters[C_KEPT];
+    c->st.train_pairs = c->h_counters[C_PAIRS];
+    c->st.train_groups = c->h_counters[C_GROUPS];
+    c->st.train_kernel_ms = ms;
+    c->st.train_ms_total += ms;
+    return FELOG_OK;
+}
+
+felog_status felog_train_round(felog_ctx *c, const felog_corpus *corpus) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->corpus.valid || !c->corpus.from_walk)
+        return fail(c, FELOG_ERR_STATE, "no walk corpus: call felog_walk_round first");
+    if (corpus && corpus->round != c->corpus.round)
+        return fail(c, FELOG_ERR_ARG, "corpus handle is not the context's current corpus");
+    const uint32_t round = c->corpus.round;
+    s = train_device_corpus(c, c->d_off, c->d_tok, c->corpus.n_walks, round, round_alpha(c->opt, round));
+    if (s) return s;
+    c->rounds_done = round;
+    return FELOG_OK;
+}
+
+felog_status felog_train_corpus(felog_ctx *c, const uint64_t *offsets, const uint32_t *tokens, uint64_t n,
+                                uint32_t round) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (round == 0) return fail(c, FELOG_ERR_ARG, "rounds are numbered from 1");
+    if (n > 0 && !offsets) return fail(c, FELOG_ERR_ARG, "null offsets");
+    if (n == 0) return FELOG_OK;
+    const uint64_t nt = offsets[n];
+    if (offsets[0] != 0) return fail(c, FELOG_ERR_ARG, "offsets[0] must be 0");
+    for (uint64_t i = 0; i < n; ++i)
+        if (offsets[i + 1] < offsets[i]) return fail(c, FELOG_ERR_ARG, "offsets must be non-decreasing");
+    if (nt > 0 && !tokens) return fail(c, FELOG_ERR_ARG, "null tokens");
+    for (uint64_t i = 0; i < nt; ++i)
+        if (tokens[i] >= c->V) return fail(c, FELOG_ERR_ARG, "token id >= v_num");
+    uint64_t *d_off = nullptr;
+    uint32_t *d_tok = nullptr;
+    TRY(c, hipMalloc(&d_off, (n + 1) * 8));
+    TRY(c, hipMalloc(&d_tok, (nt ? nt : 1) * 4));
+    TRY(c, hipMemcpy(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
+    if (nt) TRY(c, hipMemcpy(d_tok, tokens, nt * 4, hipMemcpyHostToDevice));
+    s = train_device_corpus(c, d_off, d_tok, n, round, round_alpha(c->opt, round));
+    dfree(d_off);
+    dfree(d_tok);
+    return s;
+}
+
+felog_status felog_debug_sgns(felog_ctx *c, const uint64_t *offsets, const uint32_t *tokens, uint64_t n,
+                              uint32_t round, float alpha, uint32_t max_groups, uint32_t *n_groups,
+                              uint32_t *groups_out, uint32_t *rows_out, float *logits_out, float *dv_out,
+                              float *du_out) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!offsets || !n_groups || !groups_out || !rows_out || !logits_out || !dv_out || !du_out || max_groups == 0)
+        return fail(c, FELOG_ERR_ARG, "null debug buffer");
+    const uint64_t nt = offsets[n];
+    for (uint64_t i = 0; i < nt; ++i)
+        if (tokens[i] >= c->V) return fail(c, FELOG_ERR_ARG, "token id >= v_num");
+    uint64_t *d_off = nullptr;
+    uint32_t *d_tok = nullptr;
+    SgnsDebug dbg;
+    dbg.max_groups = max_groups;
+    TRY(c, hipMalloc(&d_off, (n + 1) * 8));
+    TRY(c, hipMalloc(&d_tok, (nt ? nt : 1) * 4));
+    TRY(c, hipMemcpy(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
+    if (nt) TRY(c, hipMemcpy(d_tok, tokens, nt * 4, hipMemcpyHostToDevice));
+    const uint64_t G = max_groups;
+    TRY(c, hipMalloc(&dbg.groups, G * 5 * 4));
+    TRY(c, hipMalloc(&dbg.rows, G * 96 * 4));
+    TRY(c, hipMalloc(&dbg.logits, G * 64 * 32 * 4));
+    TRY(c, hipMalloc(&dbg.dv, G * 64 * c->d * 4));
+    TRY(c, hipMalloc(&dbg.du, G * 32 * c->d * 4));
+    TRY(c, hipMemset(dbg.logits, 0, G * 64 * 32 * 4));
+    TRY(c, hipMemset(dbg.dv, 0, G * 64 * c->d * 4));
+    TRY(c, hipMemset(dbg.du, 0, G * 32 * c->d * 4));
+    float ms = 0.f;
+    TRY(c, run_sgns(c, d_off, d_tok, n, round, alpha, &dbg, &ms));
+    const uint64_t ng = c->h_counters[C_DEBUG];
+    *n_groups = (uint32_t)(ng < G ? ng : G);
+    const uint64_t k = *n_groups;
+    TRY(c, hipMemcpy(groups_out, dbg.groups, k * 5 * 4, hipMemcpyDeviceToHost));
+    TRY(c, hipMemcpy(rows_out, dbg.rows, k * 96 * 4, hipMemcpyDeviceToHost));
+    TRY(c, hipMemcpy(logits_out, dbg.logits, k * 64 * 32 * 4, hipMemcpyDeviceToHost));
+    TRY(c, hipMemcpy(dv_out, dbg.dv, k * 64 * c->d * 4, hipMemcpyDeviceToHost));
+    TRY(c, hipMemcpy(du_out, dbg.du, k * 32 * c->d * 4, hipMemcpyDeviceToHost));
+    dfree(d_off); dfree(d_tok);
+    dfree(dbg.groups); dfree(dbg.rows); dfree(dbg.logits); dfree(dbg.dv); dfree(dbg.du);
+    if (ng > G) return fail(c, FELOG_ERR_CAPACITY, "more groups than max_groups");
+    return FELOG_OK;
+}
+
+felog_status felog_evaluate(felog_ctx *c, uint32_t *n_unconverged, int *stop) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    float ms = 0.f;
+    TRY(c, run_eval(c, c->rounds_done, &ms));
+    const uint32_t eva = (uint32_t)c->h_counters[C_EVALNUM];
+    // src/word2vec.cu:2073-2078: stop once the unconverged count shrinks by
+    // less than 30% in a round.
+    const float ratio = c->last_eva_num ? (float)eva / (float)c->last_eva_num : 0.f;
+    const bool st = c->last_eva_num != 0 && ratio > 0.7f;
+    c->last_eva_num = eva;
+    c->n_unconverged = eva;
+    c->stop = st;
+    c->st.n_unconverged = eva;
+    c->st.eval_kernel_ms = ms;
+    c->st.eval_ms_total += ms;
+    if (n_unconverged) *n_unconverged = eva;
+    if (stop) *stop = st ? 1 : 0;
+    return FELOG_OK;
+}
+
+felog_status felog_run(felog_ctx *c) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    for (uint32_t r = c->rounds_done + 1;; ++r) {
+        if (c->opt.max_rounds && r > c->opt.max_rounds) break;
+        felog_corpus corpus;
+        if ((s = felog_walk_round(c, r, &corpus))) return s;
+        if ((s = felog_train_round(c, &corpus))) return s;
+        if (r >= c->opt.min_R) {
+            uint32_t nu = 0;
+            int stop = 0;
+            if ((s = felog_evaluate(c, &nu, &stop))) return s;
+            if (stop) break;
+        }
+        if (corpus.n_walks == 0) break;
+    }
+    return FELOG_OK;
+}
+
+felog_status felog_corpus_copy_out(felog_ctx *c, uint64_t *offsets, uint32_t *tokens, uint32_t *walker_ids) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->corpus.valid) return fail(c, FELOG_ERR_STATE, "no corpus");
+    const uint64_t n = c->corpus.n_walks;
+    if (offsets) TRY(c, hipMemcpy(offsets, c->d_off, (n + 1) * 8, hipMemcpyDeviceToHost));
+    if (tokens && c->corpus.n_tokens) TRY(c, hipMemcpy(tokens, c->d_tok, c->corpus.n_tokens * 4, hipMemcpyDeviceToHost));
+    if (walker_ids && n) TRY(c, hipMemcpy(walker_ids, c->d_sent_id, n * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+felog_status felog_csr_copy_out(felog_ctx *c, uint64_t *row_ptr, uint32_t *col, float *w) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (row_ptr) TRY(c, hipMemcpy(row_ptr, c->d_row_ptr, ((uint64_t)c->V + 1) * 8, hipMemcpyDeviceToHost));
+    if (col && c->E) TRY(c, hipMemcpy(col, c->d_col, c->E * 4, hipMemcpyDeviceToHost));
+    if (w && c->E) TRY(c, hipMemcpy(w, c->d_w, c->E * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+felog_status felog_alias_copy_out(felog_ctx *c, uint32_t *thresh, uint32_t *alias_idx) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->E) return FELOG_OK;
+    std::vector<EdgeRec> rec(c->E);
+    TRY(c, hipMemcpy(rec.data(), c->d_erec, c->E * sizeof(EdgeRec), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < c->E; ++i) {
+        if (thresh) thresh[i] = rec[i].thresh;
+        if (alias_idx) alias_idx[i] = rec[i].alias_idx;
+    }
+    return FELOG_OK;
+}
+
+felog_status felog_vocab_copy_out(felog_ctx *c, uint32_t *order, uint32_t *degree) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (order) TRY(c, hipMemcpy(order, c->d_vocab_order, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    if (degree) TRY(c, hipMemcpy(degree, c->d_deg, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+felog_status felog_unigram_copy_out(felog_ctx *c, uint32_t *thresh, uint32_t *alias_idx) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (thresh) TRY(c, hipMemcpy(thresh, c->d_uni_thr, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    if (alias_idx) TRY(c, hipMemcpy(alias_idx, c->d_uni_alias, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+felog_status felog_flags_copy_out(felog_ctx *c, uint8_t *flags, float *scores) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (flags) TRY(c, hipMemcpy(flags, c->d_flags, c->V, hipMemcpyDeviceToHost));
+    if (scores) TRY(c, hipMemcpy(scores, c->d_scores, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+static felog_status copy_table_out(felog_ctx *c, const void *dtab, float *out) {
+    const uint64_t V = c->V, d = c->d, dp = c->dp;
+    if (c->opt.precision == FELOG_FP32) {
+        TRY(c, hipMemcpy2D(out, d * 4, dtab, dp * 4, d * 4, V, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<uint16_t> tmp(V * d);
+        TRY(c, hipMemcpy2D(tmp.data(), d * 2, dtab, dp * 2, d * 2, V, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < V * d; ++i) {
+            const uint32_t b = (uint32_t)tmp[i] << 16;
+            std::memcpy(&out[i], &b, 4);
+        }
+    }
+    return FELOG_OK;
+}
+
+static felog_status copy_table_in(felog_ctx *c, void *dtab, const float *in) {
+    const uint64_t V = c->V, d = c->d, dp = c->dp;
+    if (c->opt.precision == FELOG_FP32) {
+        TRY(c, hipMemset(dtab, 0, V * dp * 4));
+        TRY(c, hipMemcpy2D(dtab, dp * 4, in, d * 4, d * 4, V, hipMemcpyHostToDevice));
+    } else {
+        std::vector<uint16_t> tmp(V * dp, 0);
+        for (uint64_t v = 0; v < V; ++v)
+            for (uint64_t j = 0; j < d; ++j) {
+                uint32_t b;
+                std::memcpy(&b, &in[v * d + j], 4);
+                b += 0x7FFFu + ((b >> 16) & 1u);  // round to nearest even
+                tmp[v * dp + j] = (uint16_t)(b >> 16);
+            }
+        TRY(c, hipMemcpy(dtab, tmp.data(), V * dp * 2, hipMemcpyHostToDevice));
+    }
+    return FELOG_OK;
+}
+
+felog_status felog_embeddings_copy_out(felog_ctx *c, float *syn0, float *syn1neg) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (syn0 && (s = copy_table_out(c, c->d_syn0, syn0))) return s;
+    if (syn1neg && (s = copy_table_out(c, c->d_syn1, syn1neg))) return s;
+    return FELOG_OK;
+}
+
+felog_status felog_embeddings_copy_in(felog_ctx *c, const float *syn0, const float *syn1neg) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (syn0 && (s = copy_table_in(c, c->d_syn0, syn0))) return s;
+    if (syn1neg && (s = copy_table_in(c, c->d_syn1, syn1neg))) return s;
+    return FELOG_OK;
+}
+
+// Walk file of include/path.hpp:87-97 (the reference's writer is disabled):
+// <prefix>-<rank>-<round>.txt, one line per non-empty walk, each vertex as
+// " %u", in walker order.
+felog_status felog_write_walks(felog_ctx *c, const char *prefix) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!prefix) return fail(c, FELOG_ERR_ARG, "null prefix");
+    if (!c->corpus.valid || !c->corpus.from_walk) return fail(c, FELOG_ERR_STATE, "no walk corpus");
+    const uint64_t n = c->corpus.n_walks;
+    std::vector<uint64_t> off(n + 1);
+    std::vector<uint32_t> tok(c->corpus.n_tokens);
+    if ((s = felog_corpus_copy_out(c, off.data(), tok.data(), nullptr))) return s;
+    const std::string path = std::string(prefix) + "-0-" + std::to_string(c->corpus.round) + ".txt";
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f) return fail(c, FELOG_ERR_IO, "cannot open " + path);
+    for (uint64_t i = 0; i < n; ++i) {
+        if (off[i + 1] == off[i]) continue;
+        for (uint64_t k = off[i]; k < off[i + 1]; ++k) std::fprintf(f, " %u", tok[k]);
+        std::fputc('\n', f);
+    }
+    if (std::fclose(f) != 0) return fail(c, FELOG_ERR_IO, "write failed: " + path);
+    return FELOG_OK;
+}
+
+// Embedding file of src/word2vec.cu:2117-2127: "V d\n", then one row per
+// vocabulary entry in vocabulary order: "<vertex id> " followed by d values
+// as "%lf " (or d raw f32 with binary), then "\n".
+felog_status felog_write_embeddings(felog_ctx *c, const char *path, int binary) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!path) return fail(c, FELOG_ERR_ARG, "null path");
+    const uint64_t V = c->V, d = c->d;
+    std::vector<float> syn0(V * d);
+    if ((s = felog_embeddings_copy_out(c, syn0.data(), nullptr))) return s;
+    std::vector<uint32_t> order(V), deg(V);
+    if ((s = felog_vocab_copy_out(c, order.data(), deg.data()))) return s;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(c, FELOG_ERR_IO, std::string("cannot open ") + path);
+    std::fprintf(f, "%lld %lld\n", (long long)c->vocab_size, (long long)d);
+    for (uint64_t i = 0; i < V; ++i) {
+        const uint32_t v = order[i];
+        if (!(deg[v] >= c->opt.min_count || i == 0)) continue;
+        std::fprintf(f, "%u ", v);
+        if (binary) std::fwrite(&syn0[(uint64_t)v * d], 4, d, f);
+        else
+            for (uint64_t j = 0; j < d; ++j) std::fprintf(f, "%lf ", (double)syn0[(uint64_t)v * d + j]);
+        std::fputc('\n', f);
+    }
+    if (std::fclose(f) != 0) return fail(c, FELOG_ERR_IO, std::string("write failed: ") + path);
+    return FELOG_OK;
+}
+
+felog_status felog_stats_get(felog_ctx *c, felog_stats *out) {
+    if (!c || !out) return FELOG_ERR_ARG;
+    *out = c->st;
+    out->rounds_done = c->rounds_done;
+    return FELOG_OK;
+}
+
+}  // extern "C"

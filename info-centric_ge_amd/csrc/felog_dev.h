@@ -1,0 +1,120 @@
+// felog_dev.h - device-side building blocks shared by the HIP kernels of
+// libfelog_hip.so: Philox4x32-10, the counter layout of every random draw,
+// alias sampling, and the FP64 entropy/correlation update of a walker.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace felog {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11).  Counter-based: every draw of the hot path is a
+// pure function of (seed, counter), so the host can replay any walk.
+// ---------------------------------------------------------------------------
+struct u32x4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// Counter layout {x: index, y: domain | sub-index, z: stream, w: round}.
+// Domains keep the walk, subsampling, window, negative, rounding and
+// evaluation streams disjoint under one key.
+enum : uint32_t {
+    DOM_WALK = 0x00000000u,       // y = resample attempt, x = step, z = walker
+    DOM_SUBSAMPLE = 0x10000000u,  // x = raw position, z = sentence
+    DOM_WINDOW = 0x20000000u,     // x = kept position of the centre, z = sentence
+    DOM_NEGATIVE = 0x30000000u,   // y |= negative slot, x = group, z = sentence
+    DOM_EVAL = 0x40000000u,       // y |= draw index, x = vertex, z = ~0
+    DOM_ROUND_BF16 = 0x50000000u, // y |= group, x = tile*64+lane, z = sentence
+};
+
+__device__ __forceinline__ u32x4 draw(uint64_t seed, uint32_t x, uint32_t y,
+                                      uint32_t z, uint32_t w) {
+    return philox4x32_10(u32x4{x, y, z, w}, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Alias draw over n buckets: bucket from the high word of r0*n, coin r1
+// against the bucket's u32 threshold.  Returns the chosen bucket index.
+__device__ __forceinline__ uint32_t alias_bucket(uint32_t r0, uint32_t n) {
+    return __umulhi(r0, n);
+}
+
+// Maximum resamples of a self-loop at step 0 before the walk is ended
+// (the reference resamples without bound, include/walk.hpp:981-983).
+constexpr uint32_t kMaxStartAttempts = 64;
+// Entries of a walker's visit history kept in LDS; longer walks search
+// the rest of their path in global memory.
+constexpr int kHistLds = 32;
+// Stop threshold on R^2 (include/walk.hpp:36).
+constexpr double kRThreshold = 0.999;
+
+// Entropy / correlation state of one walker (include/walk.hpp:272-366),
+// restated with exact arithmetic:  A = sum_c T[c] over the visit counts c,
+// H = (T[S] - A) / S is the Shannon entropy (bits) of the visit histogram
+// after S visits, and the running means follow Correlation::iterateEX.
+struct WalkStats {
+    double A, EH, ES, EHS, EHH, ESS;
+};
+
+__device__ __forceinline__ void walk_stats_init(WalkStats &st) {
+    // include/walk.hpp:937-939: (H, S, EH, ES, EHS, EHH, ESS) = (0,1,0,1,0,0,1)
+    st.A = 0.0; st.EH = 0.0; st.ES = 1.0; st.EHS = 0.0; st.EHH = 0.0; st.ESS = 1.0;
+}
+
+// Enter a vertex previously visited f times; s_new = vertices on the path
+// after the move.  Returns true when the stop rule fires at this step.
+// Every operation is a correctly rounded IEEE op with contraction disabled,
+// so the host replay reproduces each bit.
+__device__ __forceinline__ bool walk_stats_step(WalkStats &st, const double *__restrict__ T,
+                                                const double *__restrict__ dT, uint32_t f,
+                                                uint32_t s_new, uint32_t step, uint32_t min_L) {
+#pragma clang fp contract(off)
+    st.A = st.A + dT[f];
+    const double S = (double)s_new;
+    const double H = (T[s_new] - st.A) / S;
+    const double N = S;
+    st.EH = st.EH + (H - st.EH) / N;
+    st.ES = st.ES + (S - st.ES) / N;
+    st.EHS = st.EHS + (H * S - st.EHS) / N;
+    st.EHH = st.EHH + (H * H - st.EHH) / N;
+    st.ESS = st.ESS + (S * S - st.ESS) / N;
+    const double DH = st.EHH - st.EH * st.EH;
+    const double DS = st.ESS - st.ES * st.ES;
+    const double e = st.EHS - st.EH * st.ES;
+    const double den = DH * DS;
+    double R = 0.0;
+    if (e != 0.0 && den > 0.0) R = e / __builtin_sqrt(den);
+    return step > min_L && (R * R < kRThreshold || R < 0.0);
+}
+
+// bf16 helpers (round-to-nearest-even is never used for table updates; they
+// are stochastically rounded).
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+    return __uint_as_float(((uint32_t)h) << 16);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16_sr(float x, uint32_t r16) {
+    uint32_t b = __float_as_uint(x);
+    if ((b & 0x7F800000u) == 0x7F800000u) return (uint16_t)(b >> 16);  // inf/nan
+    b += (r16 & 0xFFFFu);
+    return (uint16_t)(b >> 16);
+}
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+    uint32_t b = __float_as_uint(x);
+    if ((b & 0x7F800000u) == 0x7F800000u) return (uint16_t)(b >> 16);
+    b += 0x7FFFu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+
+}  // namespace felog

@@ -1,0 +1,154 @@
+// felog_internal.h - the context behind the C ABI and the host-side launch
+// functions of each kernel family.  Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/felog.h"
+
+namespace felog {
+
+// Device record of one CSR vertex: first CSR entry and degree.
+struct VertexRec { uint32_t begin, deg; };
+// Device record of one CSR entry j of vertex v: the alias threshold of
+// column j, the neighbour in column j, the neighbour in column alias(j) and
+// alias(j) itself.  One 16-byte gather decides a walk step.
+struct EdgeRec { uint32_t thresh, col, alias_col, alias_idx; };
+
+struct Corpus {
+    uint32_t round = 0;
+    uint64_t n_walks = 0, n_tokens = 0, n_steps = 0;
+    uint32_t max_len = 0;
+    bool valid = false;
+    bool from_walk = false;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace felog
+
+struct felog_ctx {
+    felog_options opt{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t cu_count = 0;
+
+    // sizes
+    uint32_t V = 0;
+    uint64_t E = 0;         // directed CSR entries
+    uint32_t d = 0;         // -size
+    uint32_t dp = 0;        // row pitch (d rounded up to 32)
+    uint64_t vocab_size = 0, train_words = 0;
+    uint32_t slot = 0;      // per-walker fixed walk slot (vertices)
+
+    // graph input (directed edge list before the CSR is built)
+    uint32_t *d_esrc = nullptr, *d_edst = nullptr;
+    float *d_ew = nullptr;
+    bool have_edges = false, built = false;
+
+    // CSR + alias
+    uint64_t *d_row_ptr = nullptr;       // V+1
+    uint32_t *d_col = nullptr;           // E
+    float *d_w = nullptr;                // E
+    felog::VertexRec *d_vrec = nullptr;  // V
+    felog::EdgeRec *d_erec = nullptr;    // E
+    uint32_t *d_deg = nullptr;           // V
+
+    // vocabulary / sampling tables
+    uint32_t *d_vocab_order = nullptr;   // V, rank -> vertex
+    uint32_t *d_vocab_rank = nullptr;    // V, vertex -> rank
+    uint8_t *d_in_vocab = nullptr;       // V
+    uint32_t *d_uni_thr = nullptr, *d_uni_alias = nullptr;  // V
+    uint32_t *d_keep_thr = nullptr;      // V
+    double *d_T = nullptr, *d_dT = nullptr;  // n log2 n table and its differences
+
+    // embedding tables (f32 or bf16 rows of dp elements, indexed by vertex id)
+    void *d_syn0 = nullptr, *d_syn1 = nullptr;
+
+    // convergence state
+    uint8_t *d_flags = nullptr;          // V
+    float *d_scores = nullptr;           // V
+    uint32_t last_eva_num = 0;
+    uint32_t n_unconverged = 0;
+    uint32_t rounds_done = 0;
+    bool stop = false;
+
+    // walk buffers
+    uint32_t walkers = 0;
+    uint32_t *d_active = nullptr;        // walker ids of this round
+    uint8_t *d_walk_ok = nullptr;
+    uint32_t *d_slots = nullptr;         // walkers * slot
+    uint32_t *d_lens = nullptr;          // walkers
+    uint64_t *d_ovf_base = nullptr;      // walkers
+    uint32_t *d_ovf = nullptr;           // overflow pool
+    uint64_t ovf_cap = 0;
+    uint64_t *d_counters = nullptr;      // small device counters
+    uint64_t *h_counters = nullptr;      // pinned mirror
+    // flat corpus
+    uint64_t *d_off = nullptr;           // n_walks + 1
+    uint32_t *d_tok = nullptr;
+    uint32_t *d_sent_id = nullptr;       // walker id per sentence
+    uint64_t tok_cap = 0, off_cap = 0;
+    felog::Corpus corpus;
+
+    // scratch for library primitives
+    void *d_tmp = nullptr;
+    size_t tmp_bytes = 0;
+
+    felog_stats st{};
+};
+
+namespace felog {
+// Counter slots in ctx->d_counters.
+enum : int {
+    C_WORK = 0,        // persistent-kernel work cursor
+    C_STEPS = 1,       // walk steps taken
+    C_OVF_NEXT = 2,    // overflow pool cursor
+    C_OVF_FAIL = 3,    // walks that did not get overflow storage
+    C_MAXLEN = 4,      // longest walk
+    C_BADEDGE = 5,     // edges with out-of-range ids
+    C_WORDS = 6,       // SGNS tokens read
+    C_KEPT = 7,        // SGNS tokens kept
+    C_PAIRS = 8,       // SGNS (centre, context) pairs
+    C_GROUPS = 9,      // SGNS groups
+    C_SENT = 10,       // SGNS sentence cursor
+    C_EVALNUM = 11,    // vertices evaluated and not converged
+    C_EVALCONV = 12,   // vertices flagged this evaluation
+    C_DEBUG = 13,      // debug group cursor
+    kNumCounters = 16
+};
+
+// graph_kernels.hip
+hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w,
+                           uint64_t m, bool undirected);
+hipError_t build_csr(felog_ctx *c);
+hipError_t build_alias(felog_ctx *c);
+hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg_host);
+// walk_kernel.hip
+hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active, float *walk_ms);
+hipError_t compact_corpus(felog_ctx *c, uint32_t n_active, float *ms);
+// sgns_kernel.hip
+struct SgnsDebug {
+    uint32_t max_groups;
+    uint32_t *groups, *rows;
+    float *logits, *dv, *du;
+};
+hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent,
+                    uint32_t round, float alpha, const SgnsDebug *dbg, float *ms);
+// eval_kernel.hip
+hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms);
+hipError_t init_tables(felog_ctx *c);
+
+uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative);
+
+// Release device memory; a failure here has nothing left to report to.
+inline void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+}  // namespace felog

@@ -1,0 +1,293 @@
+"""Python binding of libfelog_hip.so (the C ABI declared in include/felog.h).
+
+The binding is a thin ctypes layer: every method calls one ``felog_*`` entry
+point and raises :class:`FelogError` with the library's message on a negative
+status.  There is no fallback: if the shared library is missing the import of
+:func:`load_library` raises, so nothing silently runs on the CPU.
+
+Reference interfaces mirrored (paths relative to the reference repository):
+``WalkEngine::random_walk`` (include/walk.hpp:529-542), ``train_corpus_cuda``
+(src/word2vec.cu:2193), ``GraphEngine::load_graph`` (include/graph.hpp:712).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfelog_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "felog.h")
+
+FP32, BF16 = 0, 1
+STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_IO", -3: "ERR_HIP", -4: "ERR_OOM",
+          -5: "ERR_INVARIANT", -6: "ERR_STATE", -7: "ERR_CAPACITY"}
+
+
+class FelogError(RuntimeError):
+    def __init__(self, status, where, msg):
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("v_num", C.c_uint32), ("walkers", C.c_uint32), ("min_L", C.c_uint32),
+        ("min_R", C.c_uint32), ("eval_batch", C.c_uint32), ("make_undirected", C.c_int32),
+        ("size", C.c_uint32), ("window", C.c_uint32), ("negative", C.c_uint32),
+        ("batch_size", C.c_uint32), ("sample", C.c_float), ("alpha", C.c_float),
+        ("min_count", C.c_uint32), ("binary", C.c_int32), ("precision", C.c_int32),
+        ("max_walk_len", C.c_uint32), ("overflow_capacity", C.c_uint64), ("seed", C.c_uint64),
+        ("compat_lr", C.c_int32), ("compat_vocab_break", C.c_int32), ("max_rounds", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
+class Corpus(C.Structure):
+    _fields_ = [("round", C.c_uint32), ("max_len", C.c_uint32), ("n_walks", C.c_uint64),
+                ("n_tokens", C.c_uint64), ("n_steps", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("v_num", C.c_uint64), ("e_num", C.c_uint64), ("vocab_size", C.c_uint64),
+        ("rounds_done", C.c_uint32), ("n_unconverged", C.c_uint32),
+        ("walk_steps", C.c_uint64), ("walk_tokens", C.c_uint64), ("train_words", C.c_uint64),
+        ("train_kept", C.c_uint64), ("train_pairs", C.c_uint64), ("train_groups", C.c_uint64),
+        ("overflow_used", C.c_uint64),
+        ("build_ms", C.c_double), ("walk_kernel_ms", C.c_double), ("compact_kernel_ms", C.c_double),
+        ("train_kernel_ms", C.c_double), ("eval_kernel_ms", C.c_double),
+        ("walk_ms_total", C.c_double), ("train_ms_total", C.c_double), ("eval_ms_total", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def header_symbols(path=HEADER_PATH):
+    """Names of every entry point include/felog.h declares."""
+    text = open(path).read()
+    return sorted(set(re.findall(r"\b(felog_\w+)\s*\(", text)))
+
+
+def load_library(path=LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `make -C info-centric_ge_amd`")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "felog_default_options": (None, [C.POINTER(Options)]),
+        "felog_create": (C.c_int32, [C.POINTER(P), C.POINTER(Options)]),
+        "felog_destroy": (C.c_int32, [P]),
+        "felog_last_error": (C.c_char_p, [P]),
+        "felog_load_edges": (C.c_int32, [P, C.c_char_p, C.c_uint32, C.c_int]),
+        "felog_set_edges": (C.c_int32, [P, P, P, P, C.c_uint64, C.c_uint32, C.c_int]),
+        "felog_build": (C.c_int32, [P]),
+        "felog_walk_round": (C.c_int32, [P, C.c_uint32, C.POINTER(Corpus)]),
+        "felog_train_round": (C.c_int32, [P, C.POINTER(Corpus)]),
+        "felog_train_corpus": (C.c_int32, [P, P, P, C.c_uint64, C.c_uint32]),
+        "felog_evaluate": (C.c_int32, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
+        "felog_run": (C.c_int32, [P]),
+        "felog_corpus_copy_out": (C.c_int32, [P, P, P, P]),
+        "felog_csr_copy_out": (C.c_int32, [P, P, P, P]),
+        "felog_alias_copy_out": (C.c_int32, [P, P, P]),
+        "felog_vocab_copy_out": (C.c_int32, [P, P, P]),
+        "felog_unigram_copy_out": (C.c_int32, [P, P, P]),
+        "felog_flags_copy_out": (C.c_int32, [P, P, P]),
+        "felog_embeddings_copy_out": (C.c_int32, [P, P, P]),
+        "felog_embeddings_copy_in": (C.c_int32, [P, P, P]),
+        "felog_write_walks": (C.c_int32, [P, C.c_char_p]),
+        "felog_write_embeddings": (C.c_int32, [P, C.c_char_p, C.c_int]),
+        "felog_stats_get": (C.c_int32, [P, C.POINTER(Stats)]),
+        "felog_debug_sgns": (C.c_int32, [P, P, P, C.c_uint64, C.c_uint32, C.c_float, C.c_uint32,
+                                         C.POINTER(C.c_uint32), P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_options():
+    lib = load_library()
+    o = Options()
+    lib.felog_default_options(C.byref(o))
+    return o
+
+
+class Felog:
+    """One context: one GPU's copy of the graph, tables and round state."""
+
+    def __init__(self, **overrides):
+        self.lib = load_library()
+        self.opt = default_options()
+        for k, v in overrides.items():
+            if not hasattr(self.opt, k):
+                raise KeyError(f"unknown option {k}")
+            setattr(self.opt, k, v)
+        self.ctx = C.c_void_p()
+        s = self.lib.felog_create(C.byref(self.ctx), C.byref(self.opt))
+        if s != 0:
+            raise FelogError(s, "felog_create", "invalid options or no usable device")
+        self.V = 0
+        self.E = 0
+
+    # -- helpers ---------------------------------------------------------
+    def _check(self, status, where):
+        if status != 0:
+            raise FelogError(status, where, self.lib.felog_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.felog_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- graph -----------------------------------------------------------
+    def set_edges(self, src, dst, w, v_num, make_undirected=True):
+        src = np.ascontiguousarray(src, dtype=np.uint32)
+        dst = np.ascontiguousarray(dst, dtype=np.uint32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.felog_set_edges(self.ctx, _ptr(src), _ptr(dst), _ptr(w), len(src), v_num,
+                                             int(make_undirected)), "felog_set_edges")
+        self.V = v_num
+        self.E = len(src) * (2 if make_undirected else 1)
+
+    def load_edges(self, path, v_num, make_undirected=True):
+        self._check(self.lib.felog_load_edges(self.ctx, path.encode(), v_num, int(make_undirected)),
+                    "felog_load_edges")
+        self.V = v_num
+        self.E = (os.path.getsize(path) // 12) * (2 if make_undirected else 1)
+
+    def build(self):
+        self._check(self.lib.felog_build(self.ctx), "felog_build")
+
+    # -- hot path --------------------------------------------------------
+    def walk_round(self, rnd):
+        c = Corpus()
+        self._check(self.lib.felog_walk_round(self.ctx, rnd, C.byref(c)), "felog_walk_round")
+        self._corpus = c
+        return {"round": c.round, "max_len": c.max_len, "n_walks": c.n_walks,
+                "n_tokens": c.n_tokens, "n_steps": c.n_steps}
+
+    def train_round(self):
+        self._check(self.lib.felog_train_round(self.ctx, C.byref(self._corpus)), "felog_train_round")
+
+    def train_corpus(self, offsets, tokens, rnd):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        tokens = np.ascontiguousarray(tokens, dtype=np.uint32)
+        self._check(self.lib.felog_train_corpus(self.ctx, _ptr(offsets), _ptr(tokens), len(offsets) - 1, rnd),
+                    "felog_train_corpus")
+
+    def evaluate(self):
+        nu = C.c_uint32()
+        stop = C.c_int()
+        self._check(self.lib.felog_evaluate(self.ctx, C.byref(nu), C.byref(stop)), "felog_evaluate")
+        return nu.value, bool(stop.value)
+
+    def run(self):
+        self._check(self.lib.felog_run(self.ctx), "felog_run")
+
+    # -- read-back -------------------------------------------------------
+    def corpus(self):
+        c = self._corpus
+        off = np.empty(c.n_walks + 1, np.uint64)
+        tok = np.empty(max(c.n_tokens, 1), np.uint32)
+        wid = np.empty(max(c.n_walks, 1), np.uint32)
+        self._check(self.lib.felog_corpus_copy_out(self.ctx, _ptr(off), _ptr(tok), _ptr(wid)),
+                    "felog_corpus_copy_out")
+        return off, tok[:c.n_tokens], wid[:c.n_walks]
+
+    def csr(self):
+        rp = np.empty(self.V + 1, np.uint64)
+        col = np.empty(max(self.E, 1), np.uint32)
+        w = np.empty(max(self.E, 1), np.float32)
+        self._check(self.lib.felog_csr_copy_out(self.ctx, _ptr(rp), _ptr(col), _ptr(w)), "felog_csr_copy_out")
+        return rp, col[:self.E], w[:self.E]
+
+    def alias(self):
+        t = np.empty(max(self.E, 1), np.uint32)
+        a = np.empty(max(self.E, 1), np.uint32)
+        self._check(self.lib.felog_alias_copy_out(self.ctx, _ptr(t), _ptr(a)), "felog_alias_copy_out")
+        return t[:self.E], a[:self.E]
+
+    def vocab(self):
+        order = np.empty(self.V, np.uint32)
+        deg = np.empty(self.V, np.uint32)
+        self._check(self.lib.felog_vocab_copy_out(self.ctx, _ptr(order), _ptr(deg)), "felog_vocab_copy_out")
+        return order, deg
+
+    def unigram(self):
+        t = np.empty(self.V, np.uint32)
+        a = np.empty(self.V, np.uint32)
+        self._check(self.lib.felog_unigram_copy_out(self.ctx, _ptr(t), _ptr(a)), "felog_unigram_copy_out")
+        return t, a
+
+    def flags(self):
+        f = np.empty(self.V, np.uint8)
+        s = np.empty(self.V, np.float32)
+        self._check(self.lib.felog_flags_copy_out(self.ctx, _ptr(f), _ptr(s)), "felog_flags_copy_out")
+        return f, s
+
+    def embeddings(self):
+        d = self.opt.size
+        s0 = np.empty((self.V, d), np.float32)
+        s1 = np.empty((self.V, d), np.float32)
+        self._check(self.lib.felog_embeddings_copy_out(self.ctx, _ptr(s0), _ptr(s1)), "felog_embeddings_copy_out")
+        return s0, s1
+
+    def set_embeddings(self, syn0=None, syn1=None):
+        a = None if syn0 is None else np.ascontiguousarray(syn0, dtype=np.float32)
+        b = None if syn1 is None else np.ascontiguousarray(syn1, dtype=np.float32)
+        self._check(self.lib.felog_embeddings_copy_in(self.ctx, _ptr(a), _ptr(b)), "felog_embeddings_copy_in")
+
+    def write_walks(self, prefix):
+        self._check(self.lib.felog_write_walks(self.ctx, prefix.encode()), "felog_write_walks")
+
+    def write_embeddings(self, path, binary=False):
+        self._check(self.lib.felog_write_embeddings(self.ctx, path.encode(), int(binary)),
+                    "felog_write_embeddings")
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.felog_stats_get(self.ctx, C.byref(s)), "felog_stats_get")
+        return {name: getattr(s, name) for name, _ in Stats._fields_}
+
+    def debug_sgns(self, offsets, tokens, rnd, alpha, max_groups):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        tokens = np.ascontiguousarray(tokens, dtype=np.uint32)
+        d = self.opt.size
+        G = max_groups
+        groups = np.zeros((G, 5), np.uint32)
+        rows = np.zeros((G, 96), np.uint32)
+        logits = np.zeros((G, 64, 32), np.float32)
+        dv = np.zeros((G, 64, d), np.float32)
+        du = np.zeros((G, 32, d), np.float32)
+        n = C.c_uint32()
+        self._check(self.lib.felog_debug_sgns(self.ctx, _ptr(offsets), _ptr(tokens), len(offsets) - 1, rnd,
+                                              alpha, G, C.byref(n), _ptr(groups), _ptr(rows), _ptr(logits),
+                                              _ptr(dv), _ptr(du)), "felog_debug_sgns")
+        k = n.value
+        return {"groups": groups[:k], "rows": rows[:k], "logits": logits[:k], "dv": dv[:k], "du": du[:k]}

@@ -1,0 +1,170 @@
+"""Seeded synthetic graphs, the binary edge format and the link-prediction split.
+
+Generators (all undirected, no self loops, no duplicate edges; every vertex
+gets degree >= 1 because the reference asserts it, include/walk.hpp:778):
+  erdos_renyi(n, m)          G(n, m) uniform pairs
+  rmat(n, m, a, b, c)        recursive-matrix edges, ids >= n rejected
+  chung_lu(n, m, gamma)      endpoints drawn with probability ~ expected
+                             degree, expected degrees ~ i^(-1/(gamma-1))
+Weights: "uniform" -> U(0,1], "lognormal" -> LogNormal(0, 1).
+
+Binary graph file (include/storage.hpp:40-61,107-115; src/tools/gconverter.cpp:69-76):
+packed little-endian records {u32 src, u32 dst, f32 data}, no header.
+
+Link-prediction split (src/tools/gtrain.cpp:292-343): a fraction of the
+edges is held out as positives, each only while both endpoints keep degree
+>= 2 in the remaining graph; the same number of vertex pairs that are not
+edges (and not repeated) are drawn as negatives.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+EDGE_DTYPE = np.dtype([("src", "<u4"), ("dst", "<u4"), ("data", "<f4")])
+
+
+def _weights(rng, m, law):
+    if law == "uniform":
+        return (1.0 - rng.random(m)).astype(np.float32)  # (0, 1]
+    if law == "lognormal":
+        return rng.lognormal(0.0, 1.0, m).astype(np.float32)
+    raise ValueError(law)
+
+
+def _canon(src, dst, n):
+    lo = np.minimum(src, dst).astype(np.uint64)
+    hi = np.maximum(src, dst).astype(np.uint64)
+    keep = lo != hi
+    key = lo[keep] * np.uint64(n) + hi[keep]
+    return key
+
+
+def _dedupe_keys(keys, rng):
+    u, first = np.unique(keys, return_index=True)
+    # restore generation order of first occurrences
+    order = np.argsort(first, kind="stable")
+    return u[order]
+
+
+def _finish(keys, n, rng, weights):
+    src = (keys // np.uint64(n)).astype(np.uint32)
+    dst = (keys % np.uint64(n)).astype(np.uint32)
+    # random orientation, like an edge list read from a file
+    flip = rng.random(len(src)) < 0.5
+    s = np.where(flip, dst, src)
+    d = np.where(flip, src, dst)
+    s, d = _patch_isolated(s, d, n, rng)
+    w = _weights(rng, len(s), weights)
+    return s.astype(np.uint32), d.astype(np.uint32), w
+
+
+def _patch_isolated(src, dst, n, rng):
+    deg = np.bincount(src, minlength=n) + np.bincount(dst, minlength=n)
+    iso = np.nonzero(deg == 0)[0].astype(np.uint32)
+    if len(iso) == 0:
+        return src, dst
+    nonzero = np.nonzero(deg > 0)[0]
+    if len(nonzero) == 0:
+        partner = (iso + 1) % n
+    else:
+        partner = nonzero[rng.integers(0, len(nonzero), len(iso))].astype(np.uint32)
+    return np.concatenate([src, iso]), np.concatenate([dst, partner])
+
+
+def erdos_renyi(n, m, seed=1, weights="uniform"):
+    rng = np.random.default_rng(seed)
+    keys = np.empty(0, np.uint64)
+    while len(keys) < m:
+        need = int((m - len(keys)) * 1.1) + 16
+        k = _canon(rng.integers(0, n, need, dtype=np.uint64), rng.integers(0, n, need, dtype=np.uint64), n)
+        keys = _dedupe_keys(np.concatenate([keys, k]), rng)
+    return _finish(keys[:m], n, rng, weights)
+
+
+def _rmat_batch(rng, count, scale, a, b, c):
+    src = np.zeros(count, np.uint64)
+    dst = np.zeros(count, np.uint64)
+    for _ in range(scale):
+        r = rng.random(count)
+        sbit = r >= a + b            # quadrants c, d
+        dbit = ((r >= a) & (r < a + b)) | (r >= a + b + c)  # quadrants b, d
+        src = (src << np.uint64(1)) | sbit.astype(np.uint64)
+        dst = (dst << np.uint64(1)) | dbit.astype(np.uint64)
+    return src, dst
+
+
+def rmat(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform"):
+    rng = np.random.default_rng(seed)
+    scale = int(np.ceil(np.log2(max(n, 2))))
+    keys = np.empty(0, np.uint64)
+    while len(keys) < m:
+        need = int((m - len(keys)) * 1.3) + 1024
+        s, d = _rmat_batch(rng, need, scale, a, b, c)
+        ok = (s < n) & (d < n)
+        keys = _dedupe_keys(np.concatenate([keys, _canon(s[ok], d[ok], n)]), rng)
+    return _finish(keys[:m], n, rng, weights)
+
+
+def chung_lu(n, m, gamma=2.2, seed=1, weights="uniform"):
+    rng = np.random.default_rng(seed)
+    ew = np.power(np.arange(1, n + 1, dtype=np.float64), -1.0 / (gamma - 1.0))
+    p = ew / ew.sum()
+    cdf = np.cumsum(p)
+    cdf[-1] = 1.0
+    perm = rng.permutation(n).astype(np.uint64)  # hubs at random ids
+    keys = np.empty(0, np.uint64)
+    while len(keys) < m:
+        need = int((m - len(keys)) * 1.3) + 1024
+        s = perm[np.searchsorted(cdf, rng.random(need), side="right").clip(0, n - 1)]
+        d = perm[np.searchsorted(cdf, rng.random(need), side="right").clip(0, n - 1)]
+        keys = _dedupe_keys(np.concatenate([keys, _canon(s, d, n)]), rng)
+    return _finish(keys[:m], n, rng, weights)
+
+
+def write_bin(path, src, dst, w):
+    rec = np.empty(len(src), EDGE_DTYPE)
+    rec["src"], rec["dst"], rec["data"] = src, dst, w
+    rec.tofile(path)
+
+
+def read_bin(path):
+    rec = np.fromfile(path, EDGE_DTYPE)
+    return rec["src"].copy(), rec["dst"].copy(), rec["data"].copy()
+
+
+def lp_split(src, dst, w, n, frac=0.1, seed=7):
+    """Held-out positives and sampled non-edges (src/tools/gtrain.cpp:292-343).
+
+    Returns (train_src, train_dst, train_w, pos (k,2), neg (k,2))."""
+    rng = np.random.default_rng(seed)
+    m = len(src)
+    k = int(frac * m)
+    deg = np.bincount(src, minlength=n) + np.bincount(dst, minlength=n)
+    order = rng.permutation(m)
+    held = np.zeros(m, bool)
+    got = 0
+    for e in order:
+        if got == k:
+            break
+        u, v = src[e], dst[e]
+        if deg[u] <= 1 or deg[v] <= 1:
+            continue
+        deg[u] -= 1
+        deg[v] -= 1
+        held[e] = True
+        got += 1
+    pos = np.stack([src[held], dst[held]], 1)
+    edge_keys = set(_canon(src, dst, n).tolist())
+    neg = []
+    seen = set()
+    while len(neg) < got:
+        u, v = (int(x) for x in rng.integers(0, n, 2))
+        if u == v:
+            continue
+        key = min(u, v) * n + max(u, v)
+        if key in edge_keys or key in seen:
+            continue
+        seen.add(key)
+        neg.append((u, v))
+    keep = ~held
+    return src[keep], dst[keep], w[keep], pos.astype(np.uint32), np.asarray(neg, np.uint32).reshape(-1, 2)
