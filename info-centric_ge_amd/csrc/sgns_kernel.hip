@@ -15,19 +15,23 @@
 //                       dOut = corrM*in, dIn = corrM^T*out, meta counts)
 //
 // One workgroup (4 wavefronts) owns one sentence at a time.  The kept tokens
-// of the sentence are cut into groups of B consecutive centres that share k
-// negatives.  For a group, the distinct input rows (all tokens within the
-// windows, <= 64) and the distinct output rows (centres + negatives, <= 32)
-// are staged in LDS, and with
+// of a sentence are cut into groups of B consecutive centres; each group
+// draws k negatives that its centres share.  Consecutive groups are packed
+// into one tile while the tile has at most 32 output slots (centres + k per
+// group) and its windows span at most 64 positions; a 22-token walk with
+// B = 21 and k = 5 is a single tile.  For a tile, the distinct input rows
+// (tokens within the windows) and distinct output rows (centres and
+// negatives) are staged in LDS, and with
 //   P[r][o] = # (context r, centre o) pairs,
 //   Q[r][o] = # (context r, negative o) pairs of those centres,
 //   X = Vin * Uout^T                          (MFMA, f32 accumulate)
 //   G = alpha * (P - (P+Q) * sigma(X))        (f32, sigma clamped at +-6)
 //   dVin = G * Uout,  dUout = G^T * Vin       (MFMA, f32 accumulate)
-// which is the sum of the per-pair word2vec updates with all reads taken
-// before the group's writes.  Rows are then updated Hogwild: f32 tables by
-// hardware float atomics, bf16 tables by a 32-bit compare-and-swap of two
-// elements with stochastic rounding from Philox bits.
+// which is the sum of the per-pair word2vec updates of the tile with all
+// reads taken before the tile's writes.  Rows are then updated Hogwild: f32
+// tables by hardware float atomics, bf16 tables by a 32-bit compare-and-swap
+// of two elements with stochastic rounding from Philox bits.  Nothing orders
+// one tile's writes before another tile's reads (Hogwild).
 #include <hip/hip_runtime.h>
 #include "felog_dev.h"
 #include "felog_internal.h"
@@ -40,9 +44,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int kBlock = 256;
-constexpr int RMAX = 64;     // distinct input rows of a group
-constexpr int OMAX = 32;     // distinct output rows of a group
-constexpr int GLD = OMAX + 4;  // row pitch of the G tile (floats)
+constexpr int RMAX = 64;       // distinct input rows of a tile
+constexpr int OMAX = 32;       // output slots of a tile
+constexpr int GLD = OMAX + 4;  // row pitch of the count / G tile (words)
 constexpr int MAXKEEP = 1000;  // MAX_SENTENCE_LENGTH, src/word2vec.cu:42
 constexpr float kMaxExp = 6.f; // MAX_EXP, src/word2vec.cu:40
 
@@ -73,33 +77,32 @@ struct SgnsArgs {
 
 // LDS carve-up (bytes), identical on host and device.
 struct Lds {
-    uint32_t ldv;       // row pitch of staged rows, in elements
-    size_t o_tok, o_v, o_u, o_g, o_cnt, o_small, total;
+    uint32_t ldv;  // row pitch of staged rows, in elements
+    size_t o_tok, o_v, o_u, o_g, o_small, total;
 };
 
 __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize) {
     Lds L;
     L.ldv = dp + (esize == 4 ? 4u : 8u);  // 16-byte skew between rows
     size_t o = 0;
-    L.o_tok = o; o += (size_t)(MAXKEEP + 24) * 4;
-    L.o_v = o;   o += (size_t)RMAX * L.ldv * esize;
-    L.o_u = o;   o += (size_t)OMAX * L.ldv * esize;
-    L.o_g = o;   o += (size_t)RMAX * GLD * 4;
-    L.o_cnt = o; o += (size_t)RMAX * OMAX * 4;
+    L.o_tok = o;   o += (size_t)(MAXKEEP + 24) * 4;
+    L.o_v = o;     o += (size_t)RMAX * L.ldv * esize;
+    L.o_u = o;     o += (size_t)OMAX * L.ldv * esize;
+    L.o_g = o;     o += (size_t)RMAX * GLD * 4;
     L.o_small = o; o += 1024;
     L.total = o;
     return L;
 }
 
-struct Small {  // per-group bookkeeping
+struct Small {  // per-tile bookkeeping
     uint32_t rin_tok[RMAX];
     uint32_t rout_tok[OMAX];
+    uint32_t neg_tok[OMAX];
     uint8_t pos_rin[RMAX];
     uint8_t cen_rout[OMAX];
     uint8_t neg_rout[OMAX];
     uint8_t bshrink[OMAX];
-    uint32_t neg_tok[OMAX];
-    uint32_t n_in, n_out, sid, brk, kept, words, wave_cnt[4], dbg_slot;
+    uint32_t n_in, n_out, brk, words, wave_cnt[4], dbg_slot;
 };
 static_assert(sizeof(Small) <= 1024, "small block");
 
@@ -127,6 +130,22 @@ __device__ __forceinline__ void bf16_pair_add_sr(uint16_t *p, float d0, float d1
     }
 }
 
+// Number of consecutive groups, starting at group g0, that form one tile.
+__device__ __forceinline__ uint32_t tile_groups(uint32_t g0, uint32_t m, uint32_t B, uint32_t w, uint32_t k) {
+    const uint32_t c_start = g0 * B;
+    const uint32_t lo = c_start >= w ? c_start - w : 0;
+    uint32_t ng = 1;
+    while (true) {
+        const uint32_t g = g0 + ng;
+        if (g * B >= m) break;
+        const uint32_t ce = (g + 1) * B < m ? (g + 1) * B : m;
+        const uint32_t hi = ce + w < m ? ce + w : m;
+        if ((ce - c_start) + k * (ng + 1) > (uint32_t)OMAX || hi - lo > (uint32_t)RMAX) break;
+        ++ng;
+    }
+    return ng;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -135,22 +154,19 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
     T *Vs = (T *)(lds + L.o_v);
     T *Us = (T *)(lds + L.o_u);
     float *Gs = (float *)(lds + L.o_g);
-    uint32_t *cnt = (uint32_t *)(lds + L.o_cnt);
+    uint32_t *cnt = (uint32_t *)(lds + L.o_g);  // counts share the G tile
     Small &sm = *(Small *)(lds + L.o_small);
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t w = a.window, k = a.negative, B = a.bgroup;
     const uint32_t dp = a.dp, ldv = L.ldv;
+    const uint32_t row16 = lane & 15, grp = lane >> 4;
     T *syn0 = (T *)a.syn0;
     T *syn1 = (T *)a.syn1;
-    unsigned long long my_pairs = 0, my_groups = 0, my_words = 0, my_kept = 0;
+    unsigned long long my_pairs = 0, my_tiles = 0, my_words = 0, my_kept = 0;
 
-    while (true) {
-        if (tid == 0) sm.sid = (uint32_t)atomicAdd(&a.counters[C_SENT], 1ull);
-        __syncthreads();
-        const uint64_t sid = sm.sid;
-        if (sid >= a.n_sent) break;
+    for (uint64_t sid = blockIdx.x; sid < a.n_sent; sid += gridDim.x) {
         const uint64_t b0 = a.off[sid];
         const uint32_t n = (uint32_t)(a.off[sid + 1] - b0);
 
@@ -163,18 +179,18 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
             if (p < n) {
                 t = a.tok[b0 + p];
                 const bool iv = a.in_vocab[t] != 0;
+                const uint32_t thr = a.keep_thr[t];
                 if (a.compat_break && (!iv || a.vocab_rank[t] == 0)) {
                     status = 3;
                 } else if (!iv) {
                     status = 1;
                 } else {
-                    const uint32_t thr = a.keep_thr[t];
                     status = (thr == 0xFFFFFFFFu ||
                               draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, a.round).x < thr)
                                  ? 2 : 1;
                 }
             }
-            if (tid == 0) sm.brk = 0xFFFFFFFFu;
+            if (tid == 0) { sm.brk = 0xFFFFFFFFu; sm.words = 0xFFFFFFFFu; }
             __syncthreads();
             if (status == 3) atomicMin(&sm.brk, p);
             __syncthreads();
@@ -190,13 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
             }
             const uint32_t idx = m + before + (uint32_t)__popcll(bal & lt_mask);
             if (keep && idx < (uint32_t)MAXKEEP) stok[idx] = t;
-            // tokens read: up to the break, or up to the token that fills the sentence
-            uint32_t read_end = n < chunk + kBlock ? n : chunk + kBlock;
-            if (brk != 0xFFFFFFFFu) read_end = brk + 1;
-            if (tid == 0) sm.words = 0xFFFFFFFFu;
-            __syncthreads();
             if (keep && idx == (uint32_t)MAXKEEP - 1) sm.words = p + 1;
             __syncthreads();
+            // tokens read: to the break, to the token that fills the sentence,
+            // or to the end of the chunk
+            uint32_t read_end = n < chunk + kBlock ? n : chunk + kBlock;
+            if (brk != 0xFFFFFFFFu) read_end = brk + 1;
             if (sm.words != 0xFFFFFFFFu) read_end = sm.words;
             words = read_end;
             m += total;
@@ -208,23 +223,26 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
         my_kept += m;
         if (m < 2) continue;
 
-        // ---- groups of B centres ------------------------------------------
-        for (uint32_t c0 = 0; c0 < m; c0 += B) {
-            const uint32_t nc = (m - c0) < B ? (m - c0) : B;
-            const uint32_t g = c0 / B;
+        // ---- tiles of consecutive groups ------------------------------------
+        for (uint32_t g0 = 0; g0 * B < m;) {
+            const uint32_t ng = tile_groups(g0, m, B, w, k);
+            const uint32_t c0 = g0 * B;
+            const uint32_t c_end = (g0 + ng) * B < m ? (g0 + ng) * B : m;
+            const uint32_t nc = c_end - c0;
+            const uint32_t nneg = k * ng;
             const uint32_t lo = c0 >= w ? c0 - w : 0;
-            const uint32_t hi = (c0 + nc + w) < m ? (c0 + nc + w) : m;
-            const uint32_t nin = hi - lo;  // <= B + 2w <= 64
+            const uint32_t hi = c_end + w < m ? c_end + w : m;
+            const uint32_t nin = hi - lo;  // <= 64
 
             if (tid < nc) sm.bshrink[tid] = (uint8_t)(draw(a.seed, c0 + tid, DOM_WINDOW, (uint32_t)sid, a.round).x % w);
-            if (tid >= 64 && tid < 64 + k) {
-                const uint32_t j = tid - 64;
-                const u32x4 r = draw(a.seed, g, DOM_NEGATIVE | j, (uint32_t)sid, a.round);
+            if (tid >= 64 && tid < 64 + nneg) {
+                const uint32_t q = tid - 64, gq = g0 + q / k, j = q % k;
+                const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, a.round);
                 const uint32_t bkt = alias_bucket(r.x, a.V);
-                sm.neg_tok[j] = r.y < a.uni_thr[bkt] ? bkt : a.uni_alias[bkt];
+                sm.neg_tok[q] = r.y < a.uni_thr[bkt] ? bkt : a.uni_alias[bkt];
             }
-            // zero the count matrix
-            for (uint32_t i = tid; i < (uint32_t)(RMAX * OMAX); i += kBlock) cnt[i] = 0;
+            for (uint32_t i = tid; i < (uint32_t)(RMAX * GLD); i += kBlock) cnt[i] = 0;
+            if (a.dbg_max && tid == 0) sm.dbg_slot = (uint32_t)atomicAdd(&a.counters[C_DEBUG], 1ull);
             __syncthreads();
             if (wave == 0) {
                 // distinct input rows over positions [lo, hi)
@@ -243,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                 if (lane == 0) sm.n_in = (uint32_t)__popcll(bal);
             } else if (wave == 1) {
                 // distinct output rows over [centres..., negatives...]
-                const uint32_t nl = nc + k;
+                const uint32_t nl = nc + nneg;
                 const bool valid = lane < nl;
                 uint32_t t = 0xFFFFFFFFu;
                 if (valid) t = lane < nc ? stok[c0 + lane] : sm.neg_tok[lane - nc];
@@ -277,10 +295,11 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                 const int p = (int)(c0 + i) - (int)w + (int)aa;
                 if (p < 0 || p >= (int)m) continue;
                 const uint32_t rin = sm.pos_rin[p - (int)lo];
-                atomicAdd(&cnt[rin * OMAX + sm.cen_rout[i]], 1u);
+                atomicAdd(&cnt[rin * GLD + sm.cen_rout[i]], 1u);
                 const uint32_t ct = stok[c0 + i];
+                const uint32_t nb = ((c0 + i) / B - g0) * k;  // negatives of this centre's group
                 for (uint32_t j = 0; j < k; ++j)
-                    if (sm.neg_tok[j] != ct) atomicAdd(&cnt[rin * OMAX + sm.neg_rout[j]], 1u << 16);
+                    if (sm.neg_tok[nb + j] != ct) atomicAdd(&cnt[rin * GLD + sm.neg_rout[nb + j]], 1u << 16);
                 ++my_pairs;
             }
             // stage rows (16-byte vectors); padding rows are zero
@@ -304,8 +323,19 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
             }
             __syncthreads();
 
-            // ---- X = Vin * Uout^T, G = alpha (P - (P+Q) sigma(X)) -------------
-            const uint32_t row16 = lane & 15, grp = lane >> 4;
+            const uint32_t slot_dbg = a.dbg_max ? sm.dbg_slot : 0;
+            const bool dbg_on = a.dbg_max && slot_dbg < a.dbg_max;
+            if (dbg_on) {
+                if (tid == 0) {
+                    uint32_t *gq = a.dbg_groups + (uint64_t)slot_dbg * 5;
+                    gq[0] = (uint32_t)sid; gq[1] = c0; gq[2] = nc; gq[3] = n_in; gq[4] = n_out;
+                }
+                uint32_t *rr = a.dbg_rows + (uint64_t)slot_dbg * (RMAX + OMAX);
+                if (tid < RMAX) rr[tid] = tid < n_in ? sm.rin_tok[tid] : 0xFFFFFFFFu;
+                else if (tid < RMAX + OMAX) rr[tid] = (tid - RMAX) < n_out ? sm.rout_tok[tid - RMAX] : 0xFFFFFFFFu;
+            }
+
+            // ---- X = Vin * Uout^T, G = alpha (P - (P+Q) sigma(X)) in place ----
             const uint32_t tilesX = (R16 / 16) * (O16 / 16);
             for (uint32_t t = wave; t < tilesX; t += 4) {
                 const uint32_t rt = t / (O16 / 16), ot = t % (O16 / 16);
@@ -333,41 +363,13 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                 const uint32_t o = ot * 16 + row16;
                 for (int jj = 0; jj < 4; ++jj) {
                     const uint32_t r = rt * 16 + 4 * grp + jj;
-                    const uint32_t c = cnt[r * OMAX + o];
+                    const uint32_t c = cnt[r * GLD + o];
                     const float P = (float)(c & 0xFFFFu), PQ = (float)((c & 0xFFFFu) + (c >> 16));
                     Gs[r * GLD + o] = a.alpha * (P - PQ * sigmoid_clamped(acc[jj]));
-                }
-                if (a.dbg_max) {
-                    // the counts are consumed; their slots carry the logits to
-                    // the debug buffer
-                    for (int jj = 0; jj < 4; ++jj) {
-                        const uint32_t r = rt * 16 + 4 * grp + jj;
-                        cnt[r * OMAX + o] = __float_as_uint(acc[jj]);
-                    }
+                    if (dbg_on) a.dbg_logits[((uint64_t)slot_dbg * RMAX + r) * OMAX + o] = acc[jj];
                 }
             }
             __syncthreads();
-
-            uint32_t slot_dbg = 0;
-            if (a.dbg_max) {
-                if (tid == 0) sm.dbg_slot = (uint32_t)atomicAdd(&a.counters[C_DEBUG], 1ull);
-                __syncthreads();
-                slot_dbg = sm.dbg_slot;
-                if (slot_dbg < a.dbg_max) {
-                    if (tid == 0) {
-                        uint32_t *gq = a.dbg_groups + (uint64_t)slot_dbg * 5;
-                        gq[0] = (uint32_t)sid; gq[1] = c0; gq[2] = nc; gq[3] = n_in; gq[4] = n_out;
-                    }
-                    uint32_t *rr = a.dbg_rows + (uint64_t)slot_dbg * (RMAX + OMAX);
-                    if (tid < RMAX) rr[tid] = tid < n_in ? sm.rin_tok[tid] : 0xFFFFFFFFu;
-                    else if (tid < RMAX + OMAX) rr[tid] = (tid - RMAX) < n_out ? sm.rout_tok[tid - RMAX] : 0xFFFFFFFFu;
-                    float *lg = a.dbg_logits + (uint64_t)slot_dbg * RMAX * OMAX;
-                    for (uint32_t i = tid; i < (uint32_t)(RMAX * OMAX); i += kBlock) {
-                        const uint32_t r = i / OMAX, o = i % OMAX;
-                        lg[i] = (r < R16 && o < O16) ? __uint_as_float(cnt[i]) : 0.f;
-                    }
-                }
-            }
 
             // ---- dVin = G * Uout  (R16 x dp, K = O16) ---------------------------
             const uint32_t dtiles = dp / 16;
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                 }
                 const uint32_t col = dt * 16 + row16;
                 if (a.dbg_max) {
-                    if (slot_dbg < a.dbg_max && col < a.d)
+                    if (dbg_on && col < a.d)
                         for (int jj = 0; jj < 4; ++jj) {
                             const uint32_t r = rt * 16 + 4 * grp + jj;
                             a.dbg_dv[((uint64_t)slot_dbg * RMAX + r) * a.d + col] = acc[jj];
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                     // write rows 0-1 of their 4, odd lanes rows 2-3
                     float other[4];
                     for (int jj = 0; jj < 4; ++jj) other[jj] = __shfl_xor(acc[jj], 1);
-                    const u32x4 rnd = draw(a.seed, t * 64 + lane, DOM_ROUND_BF16 | g, (uint32_t)sid, a.round);
+                    const u32x4 rnd = draw(a.seed, t * 64 + lane, DOM_ROUND_BF16 | g0, (uint32_t)sid, a.round);
                     const bool odd = lane & 1;
                     const uint32_t c2 = col & ~1u;
                     for (int h = 0; h < 2; ++h) {
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                 }
                 const uint32_t col = dt * 16 + row16;
                 if (a.dbg_max) {
-                    if (slot_dbg < a.dbg_max && col < a.d)
+                    if (dbg_on && col < a.d)
                         for (int jj = 0; jj < 4; ++jj) {
                             const uint32_t o = ot * 16 + 4 * grp + jj;
                             a.dbg_du[((uint64_t)slot_dbg * OMAX + o) * a.d + col] = acc[jj];
@@ -442,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                 } else {
                     float other[4];
                     for (int jj = 0; jj < 4; ++jj) other[jj] = __shfl_xor(acc[jj], 1);
-                    const u32x4 rnd = draw(a.seed, (tilesV + t) * 64 + lane, DOM_ROUND_BF16 | g, (uint32_t)sid, a.round);
+                    const u32x4 rnd = draw(a.seed, (tilesV + t) * 64 + lane, DOM_ROUND_BF16 | g0, (uint32_t)sid, a.round);
                     const bool odd = lane & 1;
                     const uint32_t c2 = col & ~1u;
                     for (int h = 0; h < 2; ++h) {
@@ -456,14 +458,13 @@ __global__ __launch_bounds__(kBlock) void k_sgns(SgnsArgs a) {
                     }
                 }
             }
-            ++my_groups;
-            // make this group's row updates visible to the next group's loads
-            __threadfence();
-            __syncthreads();
+            ++my_tiles;
+            g0 += ng;
+            __syncthreads();  // the next tile reuses the LDS
         }
     }
     if (tid == 0) {
-        atomicAdd(&a.counters[C_GROUPS], my_groups);
+        atomicAdd(&a.counters[C_GROUPS], my_tiles);
         atomicAdd(&a.counters[C_WORDS], my_words);
         atomicAdd(&a.counters[C_KEPT], my_kept);
     }
@@ -531,7 +532,7 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
     uint64_t blocks = (uint64_t)per_cu * c->cu_count;
     if (blocks > n_sent) blocks = n_sent;
     if (blocks < 1) blocks = 1;
-    // zero the sentence cursor and training counters only
+    // zero the training counters only
     HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_DEBUG - C_WORDS + 1), c->stream));
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     if (bf16) k_sgns<uint16_t><<<(unsigned)blocks, kBlock, L.total, c->stream>>>(a);

@@ -68,7 +68,7 @@ _lib = None
 
 def header_symbols(path=HEADER_PATH):
     """Names of every entry point include/felog.h declares."""
-    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", open(path).read(), flags=re.S)
     return sorted(set(re.findall(r"\b(felog_\w+)\s*\(", text)))
 
 

@@ -46,6 +46,17 @@ def _dedupe_keys(keys, rng):
     return u[order]
 
 
+def _fit_patches(keys, n, m):
+    """Leading generated edges such that, with one patch edge per isolated
+    vertex, the graph has about m edges."""
+    keys = keys[:m]
+    seen = np.zeros(n, bool)
+    seen[(keys // np.uint64(n)).astype(np.int64)] = True
+    seen[(keys % np.uint64(n)).astype(np.int64)] = True
+    iso = n - int(seen.sum())
+    return keys[:max(m - iso, 1)]
+
+
 def _finish(keys, n, rng, weights):
     src = (keys // np.uint64(n)).astype(np.uint32)
     dst = (keys % np.uint64(n)).astype(np.uint32)
@@ -78,7 +89,7 @@ def erdos_renyi(n, m, seed=1, weights="uniform"):
         need = int((m - len(keys)) * 1.1) + 16
         k = _canon(rng.integers(0, n, need, dtype=np.uint64), rng.integers(0, n, need, dtype=np.uint64), n)
         keys = _dedupe_keys(np.concatenate([keys, k]), rng)
-    return _finish(keys[:m], n, rng, weights)
+    return _finish(_fit_patches(keys, n, m), n, rng, weights)
 
 
 def _rmat_batch(rng, count, scale, a, b, c):
@@ -102,7 +113,7 @@ def rmat(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform"):
         s, d = _rmat_batch(rng, need, scale, a, b, c)
         ok = (s < n) & (d < n)
         keys = _dedupe_keys(np.concatenate([keys, _canon(s[ok], d[ok], n)]), rng)
-    return _finish(keys[:m], n, rng, weights)
+    return _finish(_fit_patches(keys, n, m), n, rng, weights)
 
 
 def chung_lu(n, m, gamma=2.2, seed=1, weights="uniform"):
@@ -118,7 +129,7 @@ def chung_lu(n, m, gamma=2.2, seed=1, weights="uniform"):
         s = perm[np.searchsorted(cdf, rng.random(need), side="right").clip(0, n - 1)]
         d = perm[np.searchsorted(cdf, rng.random(need), side="right").clip(0, n - 1)]
         keys = _dedupe_keys(np.concatenate([keys, _canon(s, d, n)]), rng)
-    return _finish(keys[:m], n, rng, weights)
+    return _finish(_fit_patches(keys, n, m), n, rng, weights)
 
 
 def write_bin(path, src, dst, w):

@@ -133,6 +133,15 @@ void orc_alias_1d(uint32_t n, const double *wts, uint32_t *thresh, uint32_t *ali
     free(p); free(sm); free(lg);
 }
 
+/* Negative sampler: weight deg^0.75 for vocabulary vertices, 0 otherwise
+ * (src/word2vec.cu:674-694), as an alias table over vertex ids. */
+void orc_unigram(uint32_t V, const uint32_t *deg, const uint8_t *in_vocab, uint32_t *thresh, uint32_t *alias_idx) {
+    double *wts = (double *)malloc(((uint64_t)V + 1) * sizeof(double));
+    for (uint32_t v = 0; v < V; ++v) wts[v] = in_vocab[v] ? pow((double)deg[v], 0.75) : 0.0;
+    orc_alias_1d(V, wts, thresh, alias_idx);
+    free(wts);
+}
+
 /* ---- walker statistics ----------------------------------------------- */
 void orc_xlog2x(uint32_t L, double *T, double *dT) {
     for (uint32_t n = 0; n < L + 2; ++n) T[n] = n <= 1 ? 0.0 : (double)n * log2((double)n);
@@ -373,28 +382,40 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
     double *dV = (double *)malloc((uint64_t)RMAX * d * sizeof(double));
     double *dU = (double *)malloc((uint64_t)OMAX * d * sizeof(double));
     uint32_t P[RMAX][OMAX], Q[RMAX][OMAX];
-    uint32_t ng = 0;
+    uint32_t ntile = 0;
     uint64_t pairs = 0;
     for (uint64_t sid = 0; sid < n_sent; ++sid) {
         const uint64_t b0 = offsets[sid];
         const uint32_t m = filter_sentence(p, tokens + b0, (uint32_t)(offsets[sid + 1] - b0), sid, K);
         if (m < 2) continue;
-        for (uint32_t c0 = 0; c0 < m; c0 += B) {
-            const uint32_t nc = m - c0 < B ? m - c0 : B;
-            const uint32_t g = c0 / B;
+        for (uint32_t g0 = 0; g0 * B < m;) {
+            /* Tile: consecutive groups g0.. while centres + k per group <= 32
+             * output slots and the windows span <= 64 positions. */
+            const uint32_t c0 = g0 * B;
             const uint32_t lo = c0 >= w ? c0 - w : 0;
-            const uint32_t hi = c0 + nc + w < m ? c0 + nc + w : m;
+            uint32_t ng = 1;
+            for (;;) {
+                const uint32_t gn = g0 + ng;
+                if (gn * B >= m) break;
+                const uint32_t ce = (gn + 1) * B < m ? (gn + 1) * B : m;
+                const uint32_t he = ce + w < m ? ce + w : m;
+                if ((ce - c0) + k * (ng + 1) > OMAX || he - lo > RMAX) break;
+                ++ng;
+            }
+            const uint32_t c_end = (g0 + ng) * B < m ? (g0 + ng) * B : m;
+            const uint32_t nc = c_end - c0, nneg = k * ng;
+            const uint32_t hi = c_end + w < m ? c_end + w : m;
             uint32_t bshr[OMAX], neg[OMAX], rin_tok[RMAX], rout_tok[OMAX], pos_rin[RMAX], cen_rout[OMAX], neg_rout[OMAX];
             for (uint32_t i = 0; i < nc; ++i) {
                 uint32_t r[4];
                 rnd(p->seed, c0 + i, D_WINDOW, (uint32_t)sid, p->round, r);
                 bshr[i] = r[0] % w;
             }
-            for (uint32_t j = 0; j < k; ++j) {
+            for (uint32_t q = 0; q < nneg; ++q) {
                 uint32_t r[4];
-                rnd(p->seed, g, D_NEGATIVE | j, (uint32_t)sid, p->round, r);
+                rnd(p->seed, g0 + q / k, D_NEGATIVE | (q % k), (uint32_t)sid, p->round, r);
                 const uint32_t bk = mulhi32(r[0], p->V);
-                neg[j] = r[1] < p->uni_thr[bk] ? bk : p->uni_alias[bk];
+                neg[q] = r[1] < p->uni_thr[bk] ? bk : p->uni_alias[bk];
             }
             uint32_t n_in = 0, n_out = 0;
             for (uint32_t q = lo; q < hi; ++q) {
@@ -404,7 +425,7 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
                 if (f == n_in) rin_tok[n_in++] = K[q];
                 pos_rin[q - lo] = f;
             }
-            for (uint32_t q = 0; q < nc + k; ++q) {
+            for (uint32_t q = 0; q < nc + nneg; ++q) {
                 const uint32_t t = q < nc ? K[c0 + q] : neg[q - nc];
                 uint32_t f = n_out;
                 for (uint32_t r = 0; r < n_out; ++r)
@@ -422,8 +443,9 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
                     if (pp < 0 || pp >= (int64_t)m) continue;
                     const uint32_t r = pos_rin[pp - lo];
                     P[r][cen_rout[i]]++;
+                    const uint32_t nb = ((c0 + i) / B - g0) * k;  /* this centre's group */
                     for (uint32_t j = 0; j < k; ++j)
-                        if (neg[j] != K[c0 + i]) Q[r][neg_rout[j]]++;
+                        if (neg[nb + j] != K[c0 + i]) Q[r][neg_rout[nb + j]]++;
                     ++pairs;
                 }
             for (uint32_t r = 0; r < n_in; ++r)
@@ -448,18 +470,18 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
                     dU[(uint64_t)o * d + e] = acc;
                 }
             if (mode == 0) {
-                if (ng < max_groups) {
-                    uint32_t *gq = groups_out + (uint64_t)ng * 5;
+                if (ntile < max_groups) {
+                    uint32_t *gq = groups_out + (uint64_t)ntile * 5;
                     gq[0] = (uint32_t)sid; gq[1] = c0; gq[2] = nc; gq[3] = n_in; gq[4] = n_out;
-                    uint32_t *rr = rows_out + (uint64_t)ng * (RMAX + OMAX);
+                    uint32_t *rr = rows_out + (uint64_t)ntile * (RMAX + OMAX);
                     for (uint32_t r = 0; r < RMAX; ++r) rr[r] = r < n_in ? rin_tok[r] : 0xFFFFFFFFu;
                     for (uint32_t o = 0; o < OMAX; ++o) rr[RMAX + o] = o < n_out ? rout_tok[o] : 0xFFFFFFFFu;
-                    double *lg = logits_out + (uint64_t)ng * RMAX * OMAX;
+                    double *lg = logits_out + (uint64_t)ntile * RMAX * OMAX;
                     for (uint32_t r = 0; r < RMAX; ++r)
                         for (uint32_t o = 0; o < OMAX; ++o)
                             lg[r * OMAX + o] = (r < n_in && o < n_out) ? X[r * OMAX + o] : 0.0;
-                    double *dv = dv_out + (uint64_t)ng * RMAX * d;
-                    double *du = du_out + (uint64_t)ng * OMAX * d;
+                    double *dv = dv_out + (uint64_t)ntile * RMAX * d;
+                    double *du = du_out + (uint64_t)ntile * OMAX * d;
                     memset(dv, 0, (uint64_t)RMAX * d * sizeof(double));
                     memset(du, 0, (uint64_t)OMAX * d * sizeof(double));
                     memcpy(dv, dV, (uint64_t)n_in * d * sizeof(double));
@@ -471,12 +493,13 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
                 for (uint32_t o = 0; o < n_out; ++o)
                     for (uint32_t e = 0; e < d; ++e) syn1[(uint64_t)rout_tok[o] * d + e] += dU[(uint64_t)o * d + e];
             }
-            ++ng;
+            ++ntile;
+            g0 += ng;
         }
     }
     if (pairs_out) *pairs_out = pairs;
     free(K); free(X); free(G); free(dV); free(dU);
-    return ng;
+    return ntile;
 }
 
 /* ---- word2vec skip-gram restatement --------------------------------- */
@@ -588,6 +611,7 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                         }
                         float *l2 = syn1 + (uint64_t)target * d;
                         float f = 0.f;
+#pragma omp simd reduction(+ : f)
                         for (uint32_t e = 0; e < d; ++e) f += l1[e] * l2[e];
                         float g;
                         if (f > MAX_EXP) g = (label - 1) * alpha;

@@ -37,8 +37,11 @@ int orc_build_csr(uint32_t V, uint64_t m, const uint32_t *src, const uint32_t *d
 /* Vose alias tables per vertex (FP64 normalisation, u32 thresholds). */
 void orc_build_alias(uint32_t V, const uint64_t *row_ptr, const float *w, uint32_t *thresh,
                      uint32_t *alias_idx);
-/* Alias over one weight vector (used for the unigram^0.75 sampler). */
+/* Alias over one weight vector. */
 void orc_alias_1d(uint32_t n, const double *wts, uint32_t *thresh, uint32_t *alias_idx);
+/* Negative sampler over vertex ids: deg^0.75 (libm pow) for vocabulary
+ * vertices, 0 otherwise. */
+void orc_unigram(uint32_t V, const uint32_t *deg, const uint8_t *in_vocab, uint32_t *thresh, uint32_t *alias_idx);
 
 /* n*log2(n) for n = 0..L+1 and its forward differences for f = 0..L. */
 void orc_xlog2x(uint32_t L, double *T, double *dT);

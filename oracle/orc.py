@@ -141,8 +141,13 @@ def keep_thresholds(deg, sample, train_words):
 
 
 def unigram(deg, in_vocab):
-    wts = np.where(np.asarray(in_vocab) != 0, np.power(np.asarray(deg, np.float64), 0.75), 0.0)
-    return alias_1d(wts)
+    deg = np.ascontiguousarray(deg, np.uint32)
+    inv = np.ascontiguousarray(in_vocab, np.uint8)
+    V = len(deg)
+    t = np.empty(V, np.uint32)
+    a = np.empty(V, np.uint32)
+    lib().orc_unigram(C.c_uint32(V), P(deg), P(inv), P(t), P(a))
+    return t, a
 
 
 def init_syn0(V, d, rank):

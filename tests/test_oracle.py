@@ -1,0 +1,372 @@
+"""Pins of the CPU oracle (CPU only).
+
+Each check comes from outside the oracle: published known-answer vectors,
+closed forms recomputed from first principles, brute force on small inputs,
+finite differences, or an independent library (scikit-learn).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import graphgen
+import orc
+
+
+# ---- Philox4x32-10: Random123 known-answer vectors (kat_vectors, philox4x32 10 rounds)
+@pytest.mark.parametrize("ctr,key,expect", [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+])
+def test_philox_kat(ctr, key, expect):
+    assert [int(x) for x in orc.philox(ctr, key)] == expect
+
+
+# ---- CSR: brute force adjacency, neighbours in input order after undirecting
+def test_csr_matches_brute_force():
+    rng = np.random.default_rng(0)
+    V, m = 50, 300
+    src = rng.integers(0, V, m).astype(np.uint32)
+    dst = rng.integers(0, V, m).astype(np.uint32)
+    w = rng.random(m).astype(np.float32)
+    rp, col, wo = orc.build_csr(V, src, dst, w, True)
+    adj = [[] for _ in range(V)]
+    for i in range(m):  # edge i then its reverse (include/graph.hpp:741-772)
+        adj[src[i]].append((dst[i], w[i]))
+        adj[dst[i]].append((src[i], w[i]))
+    for v in range(V):
+        got = list(zip(col[rp[v]:rp[v + 1]].tolist(), wo[rp[v]:rp[v + 1]].tolist()))
+        assert got == [(int(a), float(b)) for a, b in adj[v]]
+
+
+def test_csr_rejects_out_of_range():
+    with pytest.raises(ValueError):
+        orc.build_csr(4, np.array([0, 5], np.uint32), np.array([1, 2], np.uint32), np.ones(2, np.float32))
+
+
+# ---- alias tables: exact identity P(j) = (t_j + sum_{alias_i = j} (2^32 - t_i)) / (2^32 n)
+def alias_probs(thresh, alias_idx):
+    n = len(thresh)
+    t = thresh.astype(np.float64)
+    keep = t / 2.0 ** 32
+    p = keep / n
+    np.add.at(p, alias_idx.astype(np.int64), (1.0 - keep) / n)
+    return p
+
+
+@pytest.mark.parametrize("case", ["uniform", "skewed", "zeros", "single", "pair", "equal", "huge"])
+def test_alias_identity(case):
+    rng = np.random.default_rng(hash(case) % 2 ** 32)
+    w = {
+        "uniform": lambda: rng.random(37),
+        "skewed": lambda: rng.lognormal(0, 3, 200),
+        "zeros": lambda: np.where(rng.random(50) < 0.5, 0.0, rng.random(50)),
+        "single": lambda: np.array([0.3]),
+        "pair": lambda: np.array([0.9, 0.1]),
+        "equal": lambda: np.full(64, 0.25),
+        "huge": lambda: rng.random(5000) ** 8,
+    }[case]().astype(np.float32)
+    rp = np.array([0, len(w)], np.uint64)
+    t, a = orc.build_alias(1, rp, w)
+    p = alias_probs(t, a)
+    target = w.astype(np.float64) / w.astype(np.float64).sum()
+    assert np.all(a < len(w))
+    assert np.max(np.abs(p - target)) <= len(w) * 2.0 ** -32 + 1e-12
+
+
+def test_unigram_law():
+    rng = np.random.default_rng(6)
+    deg = rng.integers(1, 300, 400).astype(np.uint32)
+    inv = (deg >= 5).astype(np.uint8)
+    t, a = orc.unigram(deg, inv)
+    target = np.where(inv == 1, deg.astype(np.float64) ** 0.75, 0.0)
+    target /= target.sum()
+    assert np.max(np.abs(alias_probs(t, a) - target)) <= 400 * 2.0 ** -32 + 1e-12
+
+
+def test_alias_all_zero_is_uniform():
+    w = np.zeros(9, np.float32)
+    t, a = orc.build_alias(1, np.array([0, 9], np.uint64), w)
+    assert np.allclose(alias_probs(t, a), 1 / 9, atol=1e-9)
+
+
+def test_alias_sampling_chi_square():
+    """Draw columns the way the walk does (hi32(r0*n), coin r1) from Philox
+    and compare with the weights by a chi-square test."""
+    rng = np.random.default_rng(4)
+    w = rng.random(12).astype(np.float32) + 0.05
+    t, a = orc.build_alias(1, np.array([0, 12], np.uint64), w)
+    counts = np.zeros(12)
+    N = 60_000
+    for i in range(N):
+        r = orc.philox([i, 0, 7, 1], [11, 0])
+        j = (int(r[0]) * 12) >> 32
+        counts[j if int(r[1]) < int(t[j]) else int(a[j])] += 1
+    exp = N * w / w.sum()
+    chi2 = float(((counts - exp) ** 2 / exp).sum())
+    assert chi2 < 40.0  # 11 dof, p ~ 5e-5
+
+
+# ---- walker statistics: Shannon entropy and Pearson R from the histogram
+def closed_form_series(path):
+    """H_n (bits) of the visit histogram and Pearson R of (H_i, S_i), i <= n."""
+    Hs, Ss, Rs = [0.0], [1.0], []
+    counts = {path[0]: 1}
+    for n in range(2, len(path) + 1):
+        counts[path[n - 1]] = counts.get(path[n - 1], 0) + 1
+        H = -sum(c / n * math.log2(c / n) for c in counts.values())
+        Hs.append(H)
+        Ss.append(float(n))
+        h, s = np.array(Hs), np.array(Ss)
+        cov = ((h - h.mean()) * (s - s.mean())).mean()
+        den = math.sqrt(((h - h.mean()) ** 2).mean() * ((s - s.mean()) ** 2).mean())
+        Rs.append(cov / den if den > 0 else 0.0)
+    return np.array(Hs[1:]), np.array(Rs)
+
+
+def prior_visits(path):
+    """Prior occurrences of path[i] in path[:i], for i >= 1."""
+    return [path[:i].count(path[i]) for i in range(1, len(path))]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_entropy_and_pearson_closed_form(seed):
+    rng = np.random.default_rng(seed)
+    path = [int(x) for x in rng.integers(0, 6 + 3 * seed, 60)]
+    f = prior_visits(path)
+    H, R = orc.corr_exact(f)
+    Hc, Rc = closed_form_series(path)
+    assert np.allclose(H, Hc, rtol=0, atol=1e-12)
+    mask = np.abs(Rc) > 1e-6
+    assert np.allclose(R[mask], Rc[mask], rtol=1e-9, atol=1e-9)
+    Hr, Rr = orc.corr_reference(f)  # include/walk.hpp:296-366 as written
+    assert np.allclose(Hr, H, rtol=0, atol=1e-11)
+    assert np.allclose(Rr[mask], R[mask], rtol=1e-8, atol=1e-8)
+
+
+def test_entropy_special_cases():
+    # all distinct: H = log2(S)
+    H, _ = orc.corr_exact([0] * 30)
+    assert np.allclose(H, np.log2(np.arange(2, 32)), atol=1e-12)
+    # back and forth between two vertices: H of (ceil(S/2), floor(S/2))
+    path = [0, 1] * 15
+    H, _ = orc.corr_exact(prior_visits(path))
+    for i, h in enumerate(H):
+        n = i + 2
+        a, b = (n + 1) // 2, n // 2
+        assert abs(h - (-(a / n) * math.log2(a / n) - (b / n) * math.log2(b / n))) < 1e-12
+
+
+# ---- walks: graph invariants and the stop rule
+@pytest.fixture(scope="module")
+def er_graph():
+    s, d, w = graphgen.erdos_renyi(2000, 8000, seed=3)
+    rp, col, wo = orc.build_csr(2000, s, d, w, True)
+    t, a = orc.build_alias(2000, rp, wo)
+    return rp, col, wo, t, a
+
+
+def test_walks_follow_edges_and_stop_rule(er_graph):
+    rp, col, wo, t, a = er_graph
+    off, tok, wid, steps = orc.walks(2000, rp, col, t, a, None, 2000, 20, 1000, 1, 99)
+    edges = set()
+    for v in range(2000):
+        for u in col[rp[v]:rp[v + 1]]:
+            edges.add((v, int(u)))
+    assert steps == int(off[-1]) - len(wid)
+    for i in range(0, len(wid), 37):
+        p = tok[off[i]:off[i + 1]].tolist()
+        assert p[0] == wid[i] % 2000
+        assert all((p[j], p[j + 1]) in edges for j in range(len(p) - 1))
+        _, R = closed_form_series(p)
+        # first step > min_L whose R^2 < 0.999 or R < 0 ends the walk
+        stops = [s for s in range(21, len(p)) if R[s - 1] ** 2 < 0.999 or R[s - 1] < 0]
+        assert stops and stops[0] == len(p) - 1
+
+
+def test_walks_skip_flagged_and_are_deterministic(er_graph):
+    rp, col, wo, t, a = er_graph
+    flags = np.zeros(2000, np.uint8)
+    flags[::3] = 1
+    off, tok, wid, _ = orc.walks(2000, rp, col, t, a, flags, 2000, 20, 1000, 2, 5, threads=4)
+    assert not np.any(wid % 3 == 0)
+    off2, tok2, wid2, _ = orc.walks(2000, rp, col, t, a, flags, 2000, 20, 1000, 2, 5, threads=1)
+    assert np.array_equal(tok, tok2) and np.array_equal(off, off2)
+
+
+def test_walk_edge_cases():
+    # isolated vertex 2 -> length-1 walk; degree-2 vertex with itself first -> ends
+    src = np.array([0, 1, 3], np.uint32)
+    dst = np.array([1, 0, 3], np.uint32)
+    w = np.ones(3, np.float32)
+    rp, col, wo = orc.build_csr(4, src, dst, w, True)
+    t, a = orc.build_alias(4, rp, wo)
+    off, tok, wid, _ = orc.walks(4, rp, col, t, a, None, 4, 20, 1000, 1, 1)
+    lens = np.diff(off.astype(np.int64))
+    assert lens[2] == 1           # degree 0
+    assert lens[3] == 1           # self loop only: degree 2, first neighbour itself
+    assert lens[0] == 22 and lens[1] == 22
+    # walk cap
+    off, tok, wid, _ = orc.walks(4, rp, col, t, a, None, 4, 20, 7, 1, 1)
+    assert np.diff(off.astype(np.int64))[0] == 7
+
+
+# ---- vocabulary, thresholds, initialisation
+def test_vocab_order_and_thresholds():
+    rng = np.random.default_rng(2)
+    deg = rng.integers(0, 40, 500).astype(np.uint32)
+    order, rank, inv, vs, tw = orc.vocab(deg, 5)
+    assert np.array_equal(order, np.lexsort((np.arange(500), -deg.astype(np.int64))))
+    assert np.array_equal(rank[order], np.arange(500))
+    keep_mask = (deg >= 5) | (rank == 0)
+    assert np.array_equal(inv.astype(bool), keep_mask)
+    assert vs == keep_mask.sum() and tw == int(deg[keep_mask].sum())
+    keep = orc.keep_thresholds(deg, 0.01, tw)
+    sT = 0.01 * tw
+    for v in range(500):
+        c = int(deg[v])
+        if c == 0:
+            assert keep[v] == 0xFFFFFFFF
+            continue
+        prob = (math.sqrt(c / sT) + 1) * sT / c
+        assert keep[v] == (0xFFFFFFFF if prob >= 1 else int(prob * 2 ** 32))
+
+
+def test_init_syn0_lcg_closed_form():
+    d, V = 4, 6
+    rank = np.array([3, 0, 5, 1, 2, 4], np.uint32)
+    got = orc.init_syn0(V, d, rank)
+    x = 1
+    order = np.argsort(rank)
+    for a in range(V):
+        for b in range(d):
+            x = (x * 25214903917 + 11) % 2 ** 64
+            expect = np.float32(((x & 0xFFFF) / 65536 - 0.5) / d)
+            assert got[order[a], b] == expect
+
+
+# ---- SGNS FP64 oracle: gradients by finite differences of the SGNS loss
+def test_sgns_oracle_gradient_finite_difference():
+    rng = np.random.default_rng(8)
+    V, d = 12, 4
+    deg = np.full(V, 10, np.uint32)
+    order, rank, inv, vs, tw = orc.vocab(deg, 5)
+    ut, ua = orc.unigram(deg, inv)
+    tables = dict(in_vocab=inv, vocab_rank=rank, keep_thr=np.full(V, 0xFFFFFFFF, np.uint32), uni_thr=ut,
+                  uni_alias=ua)
+    tokens = np.array([1, 4, 7, 4, 9, 2, 11, 3], np.uint32)
+    offsets = np.array([0, len(tokens)], np.uint64)
+    s0 = rng.standard_normal((V, d)) * 0.5
+    s1 = rng.standard_normal((V, d)) * 0.5
+    kw = dict(d=d, window=3, negative=3, batch=4, alpha=1.0, rnd=1, seed=77)
+    ref = orc.sgns_groups(tables, offsets, tokens, s0, s1, mode=0, **kw)
+
+    # enumerate (context, centre) and (context, negative) pairs independently
+    # of the C code, from the Philox draws; the loss is
+    # -sum log sigma(v_ctx . u_centre) - sum log sigma(-v_ctx . u_neg)
+    m, w, k, B = len(tokens), 3, 3, 4
+    pairs, negs = [], []
+    ng = -(-m // B)
+    neg = {}
+    for g in range(ng):
+        for j in range(k):
+            r = orc.philox([g, 0x30000000 | j, 0, 1], [77, 0])
+            bk = (int(r[0]) * V) >> 32
+            neg[(g, j)] = bk if int(r[1]) < int(ut[bk]) else int(ua[bk])
+    for c in range(m):
+        b = int(orc.philox([c, 0x20000000, 0, 1], [77, 0])[0]) % w
+        for aa in range(b, 2 * w + 1 - b):
+            p = c - w + aa
+            if aa == w or p < 0 or p >= m:
+                continue
+            pairs.append((int(tokens[p]), int(tokens[c])))
+            for j in range(k):
+                n_ = neg[(c // B, j)]
+                if n_ != tokens[c]:
+                    negs.append((int(tokens[p]), n_))
+
+    def L(A, Bm):
+        sig = lambda x: 1 / (1 + np.exp(-x))
+        return (-sum(np.log(sig(A[i] @ Bm[o])) for i, o in pairs)
+                - sum(np.log(sig(-(A[i] @ Bm[o]))) for i, o in negs))
+
+    assert ref["pairs"] == len(pairs)
+    eps = 1e-6
+    gV = np.zeros_like(s0)
+    gU = np.zeros_like(s1)
+    for v in range(V):
+        for e in range(d):
+            for M, G in ((s0, gV), (s1, gU)):
+                M[v, e] += eps
+                lp = L(s0, s1)
+                M[v, e] -= 2 * eps
+                lm = L(s0, s1)
+                M[v, e] += eps
+                G[v, e] = (lp - lm) / (2 * eps)
+    # the oracle's tile deltas, summed per row, equal -alpha * dL (alpha = 1)
+    dV = np.zeros_like(s0)
+    dU = np.zeros_like(s1)
+    for i, g in enumerate(ref["groups"]):
+        n_in, n_out = int(g[3]), int(g[4])
+        for r in range(n_in):
+            dV[ref["rows"][i][r]] += ref["dv"][i][r]
+        for o in range(n_out):
+            dU[ref["rows"][i][64 + o]] += ref["du"][i][o]
+    assert np.allclose(dV, -gV, atol=1e-6)
+    assert np.allclose(dU, -gU, atol=1e-6)
+
+
+# ---- AUC: Mann-Whitney with ties vs scikit-learn
+def test_auc_matches_sklearn():
+    from sklearn.metrics import roc_auc_score
+    rng = np.random.default_rng(1)
+    pos = np.round(rng.normal(0.3, 1, 500), 1).astype(np.float32)
+    neg = np.round(rng.normal(0, 1, 700), 1).astype(np.float32)
+    y = np.r_[np.ones(500), np.zeros(700)]
+    assert abs(orc.auc(pos, neg) - roc_auc_score(y, np.r_[pos, neg])) < 1e-12
+
+
+# ---- evaluation: brute-force cosine
+def test_eval_brute_force(er_graph):
+    rp, col, wo, t, a = er_graph
+    rng = np.random.default_rng(5)
+    V, d = 2000, 16
+    base = rng.standard_normal(d)
+    syn0 = (base + 0.7 * rng.standard_normal((V, d))).astype(np.float32)
+    inv = np.ones(V, np.uint8)
+    eva, flags, scores = orc.evaluate(syn0, rp, col, inv, np.zeros(V, np.uint8), 3, 1)
+    for v in range(0, V, 17):
+        nb = col[rp[v]:rp[v + 1]]
+        if len(nb) > 30:
+            continue
+        x = syn0[v].astype(np.float64)
+        cs = [x @ syn0[u] / np.linalg.norm(x) / np.linalg.norm(syn0[u]) for u in nb]
+        assert abs(scores[v] - np.mean(cs)) < 1e-5
+        assert bool(flags[v]) == (np.mean(cs) > 0.65)
+    assert eva == int((scores <= 0.65).sum())
+
+
+# ---- word2vec restatement: one pair, one negative disabled, exact sigmoid
+def test_w2v_single_pair_update():
+    V, d = 3, 4
+    syn0 = np.array([[0.1, -0.2, 0.3, 0.0], [0.05, 0.1, -0.1, 0.2], [0.0, 0.0, 0.0, 0.0]], np.float32)
+    syn1 = np.array([[0.2, 0.1, 0.0, -0.1], [0.3, -0.3, 0.2, 0.1], [0.0, 0.0, 0.0, 0.0]], np.float32)
+    deg = np.array([10, 9, 8], np.uint32)
+    order, rank, inv, vs, tw = orc.vocab(deg, 1)
+    s0, s1 = syn0.copy(), syn1.copy()
+    offsets = np.array([0, 2], np.uint64)
+    tokens = np.array([0, 1], np.uint32)
+    words, pairs = orc.w2v_train(s0, s1, offsets, tokens, deg, order, rank, inv, vs, tw, window=1, negative=0,
+                                 sample=0.0, alpha=0.5, use_lut=False, threads=1)
+    assert pairs == 2 and words == 2
+    # pair (centre 0, context 1) then (centre 1, context 0), sequentially
+    a0, a1 = syn0.astype(np.float64), syn1.astype(np.float64)
+    for centre, ctx in ((0, 1), (1, 0)):
+        f = a0[ctx] @ a1[centre]
+        g = (1 - 1 / (1 + math.exp(-f))) * 0.5
+        e = g * a1[centre]
+        a1[centre] = a1[centre] + g * a0[ctx]
+        a0[ctx] = a0[ctx] + e
+    assert np.allclose(s0, a0, atol=1e-6) and np.allclose(s1, a1, atol=1e-6)
