@@ -127,7 +127,7 @@ def main():
     for r in range(1, args.warmup + 1):
         one_round(r)
     tok = steps = words = kept = pairs = 0
-    walk_ms = comp_ms = train_ms = 0.0
+    walk_ms = comp_ms = train_ms = prep_ms = tile_ms = 0.0
     t0 = time.perf_counter()
     for r in range(args.warmup + 1, args.warmup + args.steps + 1):
         c = one_round(r)
@@ -140,6 +140,8 @@ def main():
         walk_ms += s["walk_kernel_ms"]
         comp_ms += s["compact_kernel_ms"]
         train_ms += s["train_kernel_ms"]
+        prep_ms += s["sgns_prep_ms"]
+        tile_ms += s["sgns_tile_ms"]
     elapsed = time.perf_counter() - t0
 
     K = args.steps
@@ -177,6 +179,8 @@ def main():
         "walk_kernel_ms": walk_ms / K,
         "compact_kernel_ms": comp_ms / K,
         "sgns_kernel_ms": train_ms / K,
+        "sgns_prep_kernel_ms": prep_ms / K,
+        "sgns_tile_kernel_ms": tile_ms / K,
         "tokens_per_step": tok / K,
         "pairs_per_step": pairs / K,
         "walk_roofline": {"bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -87,7 +87,18 @@ typedef struct felog_options {
                                   (src/word2vec.cu:1202-1205)                */
     uint32_t max_rounds;       /* felog_run() round limit (0 -> no limit)    */
     int32_t device;            /* HIP device ordinal                         */
+    int32_t update_mode;       /* SGNS row updates: FELOG_UPDATE_*           */
 } felog_options;
+
+/* How SGNS tiles write their row deltas (Hogwild either way).
+ * ATOMIC: f32 tables get hardware float atomic adds, so no delta is lost
+ *   (as src/word2vec.cu:530,533); bf16 tables a compare-and-swap per pair of
+ *   elements with stochastic rounding.
+ * WRITEBACK: each tile stores staged-row + delta (bf16: stochastically
+ *   rounded), like word2vec's unsynchronised CPU threads; a concurrent
+ *   tile's delta to the same row can be overwritten.
+ * DEFAULT: ATOMIC for f32 tables, WRITEBACK for bf16 tables. */
+enum { FELOG_UPDATE_DEFAULT = 0, FELOG_UPDATE_ATOMIC = 1, FELOG_UPDATE_WRITEBACK = 2 };
 
 typedef struct felog_ctx felog_ctx;
 
@@ -119,6 +130,8 @@ typedef struct felog_stats {
     double train_kernel_ms;          /* last round, HIP events, SGNS kernel */
     double eval_kernel_ms;           /* last evaluation kernel              */
     double walk_ms_total, train_ms_total, eval_ms_total;  /* all rounds     */
+    double sgns_prep_ms;             /* last round: SGNS tile preparation   */
+    double sgns_tile_ms;             /* last round: SGNS tile kernel        */
 } felog_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -190,16 +203,20 @@ felog_status felog_write_embeddings(felog_ctx *ctx, const char *path,
 felog_status felog_stats_get(felog_ctx *ctx, felog_stats *out);
 
 /* ---- test hook --------------------------------------------------------- */
-/* Runs the SGNS kernel on a caller-supplied corpus with updates diverted:
- * instead of touching the tables it writes, for every group, the f32 logits
- * and the table deltas into caller buffers.  Used to compare the kernel's
- * arithmetic with an FP64 restatement on race-free inputs.
- *   groups_out: per group {sentence, first centre, n_centres, n_in, n_out}
- *   rows_out:   per group 64 input + 32 output vertex ids (0xFFFFFFFF = pad)
- *   logits_out: per group 64 x 32 f32
- *   dv_out:     per group 64 x size f32 (input-row deltas)
- *   du_out:     per group 32 x size f32 (output-row deltas)
- * max_groups bounds all buffers; *n_groups receives the count. */
+/* Runs the SGNS kernels on a caller-supplied corpus with updates diverted:
+ * instead of touching the tables they write, for every tile (a run of
+ * centre groups trained together), the f32 logits and the table deltas into
+ * caller buffers.  Used to compare the kernels' arithmetic with an FP64
+ * restatement; every tile reads the tables as they were before the call.
+ * Slot s < n_sentences holds the first tile of sentence s (all zero when the
+ * sentence keeps fewer than two tokens); later slots hold further tiles.
+ *   groups_out: per slot {sentence, first centre, n_centres, n_in, n_out}
+ *   rows_out:   per slot 64 input + 32 output vertex ids (0xFFFFFFFF = pad)
+ *   logits_out: per slot 64 x 32 f32
+ *   dv_out:     per slot 64 x size f32 (input-row deltas)
+ *   du_out:     per slot 32 x size f32 (output-row deltas)
+ * max_groups bounds all buffers; *n_groups receives the number of slots
+ * written; FELOG_ERR_CAPACITY when more were needed. */
 felog_status felog_debug_sgns(felog_ctx *ctx, const uint64_t *offsets,
                               const uint32_t *tokens, uint64_t n_sentences,
                               uint32_t round, float alpha,

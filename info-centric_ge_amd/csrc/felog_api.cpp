@@ -59,11 +59,12 @@ void free_all(felog_ctx *c) {
                     c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni_thr,
                     c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_syn1, c->d_flags,
                     c->d_scores, c->d_active, c->d_walk_ok, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
-                    c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tmp};
+                    c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp};
     for (void *p : ptrs) dfree(p);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -114,6 +115,7 @@ felog_status felog_create(felog_ctx **out, const felog_options *opt) {
     else if (o.precision != FELOG_FP32 && o.precision != FELOG_BF16) bad = "precision must be FP32 or BF16";
     else if (o.max_walk_len < 2 || o.max_walk_len > (1u << 20)) bad = "max_walk_len must be in [2, 2^20]";
     else if (!(o.alpha > 0.f)) bad = "-alpha must be > 0";
+    else if (o.update_mode < FELOG_UPDATE_DEFAULT || o.update_mode > FELOG_UPDATE_WRITEBACK) bad = "bad update_mode";
     if (bad.empty() && sgns_group_centres(o.batch_size, o.window, o.negative) < 1)
         bad = "-window/-negative leave no room for a centre in an SGNS group";
     if (!bad.empty()) {
@@ -131,6 +133,7 @@ felog_status felog_create(felog_ctx **out, const felog_options *opt) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e == hipSuccess) e = hipMalloc(&c->d_counters, kNumCounters * 8);
     if (e == hipSuccess) e = hipHostMalloc(&c->h_counters, kNumCounters * 8, hipHostMallocDefault);
     int cus = 0;
@@ -300,15 +303,19 @@ felog_status felog_walk_round(felog_ctx *c, uint32_t round, felog_corpus *out) {
 }
 
 static felog_status train_device_corpus(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n,
-                                        uint32_t round, float alpha) {
-    float ms = 0.f;
-    TRY(c, run_sgns(c, d_off, d_tok, n, round, alpha, nullptr, &ms));
+                                        uint64_t n_tok, uint32_t round, float alpha) {
+    float ms[3] = {0.f, 0.f, 0.f};
+    TRY(c, run_sgns(c, d_off, d_tok, n, n_tok, round, alpha, nullptr, ms));
     c->st.train_words = c->h_counters[C_WORDS];
     c->st.train_kept = c->h_counters[C_KEPT];
     c->st.train_pairs = c->h_counters[C_PAIRS];
     c->st.train_groups = c->h_counters[C_GROUPS];
-    c->st.train_kernel_ms = ms;
-    c->st.train_ms_total += ms;
+    c->st.train_kernel_ms = ms[0];
+    c->st.sgns_prep_ms = ms[1];
+    c->st.sgns_tile_ms = ms[2];
+    c->st.train_ms_total += ms[0];
+    if (n + c->h_counters[C_TILES] > c->tiles_cap)
+        return fail(c, FELOG_ERR_INVARIANT, "SGNS tile descriptors exceeded their bound");
     return FELOG_OK;
 }
 
@@ -320,7 +327,8 @@ felog_status felog_train_round(felog_ctx *c, const felog_corpus *corpus) {
     if (corpus && corpus->round != c->corpus.round)
         return fail(c, FELOG_ERR_ARG, "corpus handle is not the context's current corpus");
     const uint32_t round = c->corpus.round;
-    s = train_device_corpus(c, c->d_off, c->d_tok, c->corpus.n_walks, round, round_alpha(c->opt, round));
+    s = train_device_corpus(c, c->d_off, c->d_tok, c->corpus.n_walks, c->corpus.n_tokens, round,
+                            round_alpha(c->opt, round));
     if (s) return s;
     c->rounds_done = round;
     return FELOG_OK;
@@ -346,7 +354,7 @@ felog_status felog_train_corpus(felog_ctx *c, const uint64_t *offsets, const uin
     TRY(c, hipMalloc(&d_tok, (nt ? nt : 1) * 4));
     TRY(c, hipMemcpy(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
     if (nt) TRY(c, hipMemcpy(d_tok, tokens, nt * 4, hipMemcpyHostToDevice));
-    s = train_device_corpus(c, d_off, d_tok, n, round, round_alpha(c->opt, round));
+    s = train_device_corpus(c, d_off, d_tok, n, nt, round, round_alpha(c->opt, round));
     dfree(d_off);
     dfree(d_tok);
     return s;
@@ -377,12 +385,14 @@ felog_status felog_debug_sgns(felog_ctx *c, const uint64_t *offsets, const uint3
     TRY(c, hipMalloc(&dbg.logits, G * 64 * 32 * 4));
     TRY(c, hipMalloc(&dbg.dv, G * 64 * c->d * 4));
     TRY(c, hipMalloc(&dbg.du, G * 32 * c->d * 4));
+    TRY(c, hipMemset(dbg.groups, 0, G * 5 * 4));
     TRY(c, hipMemset(dbg.logits, 0, G * 64 * 32 * 4));
     TRY(c, hipMemset(dbg.dv, 0, G * 64 * c->d * 4));
     TRY(c, hipMemset(dbg.du, 0, G * 32 * c->d * 4));
-    float ms = 0.f;
-    TRY(c, run_sgns(c, d_off, d_tok, n, round, alpha, &dbg, &ms));
-    const uint64_t ng = c->h_counters[C_DEBUG];
+    float ms[3];
+    TRY(c, run_sgns(c, d_off, d_tok, n, nt, round, alpha, &dbg, ms));
+    // slots: one per sentence (empty when it has no centre) plus extra tiles
+    const uint64_t ng = n + c->h_counters[C_TILES];
     *n_groups = (uint32_t)(ng < G ? ng : G);
     const uint64_t k = *n_groups;
     TRY(c, hipMemcpy(groups_out, dbg.groups, k * 5 * 4, hipMemcpyDeviceToHost));

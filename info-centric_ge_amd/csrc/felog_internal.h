@@ -36,7 +36,7 @@ struct felog_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
     uint32_t cu_count = 0;
 
     // sizes
@@ -97,6 +97,10 @@ struct felog_ctx {
     uint64_t tok_cap = 0, off_cap = 0;
     felog::Corpus corpus;
 
+    // SGNS tile descriptors (written by the prep kernel, read by the tile kernel)
+    void *d_tiles = nullptr;
+    uint64_t tiles_cap = 0;
+
     // scratch for library primitives
     void *d_tmp = nullptr;
     size_t tmp_bytes = 0;
@@ -116,8 +120,8 @@ enum : int {
     C_WORDS = 6,       // SGNS tokens read
     C_KEPT = 7,        // SGNS tokens kept
     C_PAIRS = 8,       // SGNS (centre, context) pairs
-    C_GROUPS = 9,      // SGNS groups
-    C_SENT = 10,       // SGNS sentence cursor
+    C_GROUPS = 9,      // SGNS tiles trained
+    C_TILES = 10,      // SGNS tile descriptors written
     C_EVALNUM = 11,    // vertices evaluated and not converged
     C_EVALCONV = 12,   // vertices flagged this evaluation
     C_DEBUG = 13,      // debug group cursor
@@ -133,19 +137,22 @@ hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg_host)
 // walk_kernel.hip
 hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active, float *walk_ms);
 hipError_t compact_corpus(felog_ctx *c, uint32_t n_active, float *ms);
-// sgns_kernel.hip
+// sgns_prep.hip (driver) + sgns_tile.hip
 struct SgnsDebug {
     uint32_t max_groups;
     uint32_t *groups, *rows;
     float *logits, *dv, *du;
 };
+// ms[0] = whole training pass, ms[1] = prep kernel, ms[2] = tile kernel
 hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent,
-                    uint32_t round, float alpha, const SgnsDebug *dbg, float *ms);
+                    uint64_t n_tok, uint32_t round, float alpha, const SgnsDebug *dbg, float *ms);
 // eval_kernel.hip
 hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms);
 hipError_t init_tables(felog_ctx *c);
 
 uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative);
+// Input rows of an SGNS tile: 48 when a group's windows fit, else 64.
+uint32_t sgns_tile_rows(uint32_t centres, uint32_t window);
 
 // Release device memory; a failure here has nothing left to report to.
 inline void dfree(void *p) {

@@ -1,0 +1,266 @@
+// sgns_prep.hip - first half of an SGNS training pass, and the pass driver.
+//
+// k_sgns_prep runs one wavefront per sentence and does the integer work of
+// the skip-gram pass (reference paths relative to the reference repository):
+//   sentence filtering  src/word2vec.cu:1196-1232  (vocabulary lookup,
+//                       subsampling keep = min(1, (sqrt(c/sT)+1) sT/c) with
+//                       c = degree and the draw Philox(seed; position,
+//                       SUBSAMPLE, sentence, round), 1000-token cut, optional
+//                       break at vocabulary index 0)
+//   window shrink       src/word2vec.cu:458-465    (b = rand % window, from
+//                       Philox(seed; centre, WINDOW, sentence, round))
+//   negatives           src/word2vec.cu:496-507    (unigram^0.75 law, here an
+//                       alias draw from Philox(seed; group, NEGATIVE|j, ...))
+// and writes, per tile, the distinct input and output rows and the index
+// maps the tile kernel needs to rebuild its pair counts.  The dependent
+// gathers (tokens -> vocabulary tables -> sampler) run at full occupancy,
+// away from the MFMA kernel.
+#include <hip/hip_runtime.h>
+#include "felog_dev.h"
+#include "felog_internal.h"
+#include "sgns_common.h"
+
+namespace felog {
+namespace sgns {
+
+namespace {
+
+constexpr int kPrepWaves = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
+    __shared__ uint32_t s_tok[kPrepWaves][MAXKEEP + 8];
+    __shared__ uint32_t s_neg[kPrepWaves][OMAX];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t *stok = s_tok[wv];
+    uint32_t *sneg = s_neg[wv];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t w = a.window, k = a.negative, B = a.bgroup;
+    unsigned long long my_words = 0, my_kept = 0;
+    const uint64_t nwaves = (uint64_t)gridDim.x * kPrepWaves;
+
+    for (uint64_t sid = (uint64_t)blockIdx.x * kPrepWaves + wv; sid < a.n_sent; sid += nwaves) {
+        const uint64_t b0 = a.off[sid];
+        const uint32_t n = (uint32_t)(a.off[sid + 1] - b0);
+        // ---- filter into stok[0, m) ----
+        uint32_t m = 0, words = 0;
+        for (uint32_t chunk = 0; chunk < n; chunk += 64) {
+            const uint32_t p = chunk + lane;
+            int status = 0;  // 0 none, 1 drop, 2 keep, 3 break
+            uint32_t t = 0;
+            if (p < n) {
+                t = a.tok[b0 + p];
+                const bool iv = a.in_vocab[t] != 0;
+                const uint32_t thr = a.keep_thr[t];
+                if (a.compat_break && (!iv || a.vocab_rank[t] == 0)) status = 3;
+                else if (!iv) status = 1;
+                else
+                    status = (thr == 0xFFFFFFFFu ||
+                              draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, a.round).x < thr) ? 2 : 1;
+            }
+            const uint64_t brk_bal = __ballot(status == 3);
+            const uint32_t brk = brk_bal ? chunk + (uint32_t)__ffsll((long long)brk_bal) - 1 : 0xFFFFFFFFu;
+            const bool keep = status == 2 && p < brk;
+            const uint64_t bal = __ballot(keep);
+            const uint32_t idx = m + (uint32_t)__popcll(bal & lt_mask);
+            if (keep && idx < (uint32_t)MAXKEEP) stok[idx] = t;
+            uint32_t read_end = n < chunk + 64 ? n : chunk + 64;
+            if (brk != 0xFFFFFFFFu) read_end = brk + 1;
+            // the token that fills the sentence ends the read
+            const uint64_t fill = __ballot(keep && idx == (uint32_t)MAXKEEP - 1);
+            if (fill) read_end = chunk + (uint32_t)__ffsll((long long)fill);
+            words = read_end;
+            m += (uint32_t)__popcll(bal);
+            if (brk != 0xFFFFFFFFu || m >= (uint32_t)MAXKEEP) break;
+        }
+        if (m > (uint32_t)MAXKEEP) m = MAXKEEP;
+        my_words += words;
+        my_kept += m;
+        if (m < 2) {
+            // slot sid stays empty: no centre, nothing to train
+            if (lane == 0) {
+                TileDesc *td = (TileDesc *)(a.tiles + sid * kDescBytes);
+                td->nc = 0;
+                td->n_in = 0;
+            }
+            continue;
+        }
+        __builtin_amdgcn_wave_barrier();
+
+        // ---- tiles ----
+        for (uint32_t g0 = 0; g0 * B < m;) {
+            const uint32_t ng = tile_groups(g0, m, B, w, k, a.rmax);
+            const uint32_t c0 = g0 * B;
+            const uint32_t c_end = (g0 + ng) * B < m ? (g0 + ng) * B : m;
+            const uint32_t nc = c_end - c0, nneg = k * ng;
+            const uint32_t lo = c0 >= w ? c0 - w : 0;
+            const uint32_t hi = c_end + w < m ? c_end + w : m;
+            const uint32_t npos = hi - lo;
+            // the first tile of sentence sid owns slot sid; further tiles take
+            // slots after the n_sent first ones
+            uint64_t slot = sid;
+            if (g0 > 0) {
+                if (lane == 0) slot = a.n_sent + atomicAdd(&a.counters[C_TILES], 1ull);
+                slot = __shfl(slot, 0);
+            }
+            // negatives of the tile's groups
+            if (lane < nneg) {
+                const uint32_t gq = g0 + lane / k, j = lane % k;
+                const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, a.round);
+                const uint32_t bkt = alias_bucket(r.x, a.V);
+                sneg[lane] = r.y < a.uni_thr[bkt] ? bkt : a.uni_alias[bkt];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (slot >= a.tiles_cap) {
+                g0 += ng;
+                continue;  // counted; the host reports the overflow
+            }
+            TileDesc *td = (TileDesc *)(a.tiles + slot * kDescBytes);
+            // distinct input rows over positions [lo, hi)
+            {
+                const bool valid = lane < npos;
+                const uint32_t t = valid ? stok[lo + lane] : 0xFFFFFFFFu;
+                uint32_t first = lane;
+                if (valid)
+                    for (uint32_t q = 0; q < lane; ++q)
+                        if (stok[lo + q] == t) { first = q; break; }
+                const bool isf = valid && first == lane;
+                const uint64_t bal = __ballot(isf);
+                const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
+                const uint32_t my = __shfl(idx, (int)first);
+                const uint32_t n_in = (uint32_t)__popcll(bal);
+                if (isf) td->rin_tok[idx] = t;
+                if (lane >= n_in) td->rin_tok[lane] = 0xFFFFFFFFu;
+                td->pos_rin[lane] = valid ? (uint8_t)my : 0;
+                if (lane == 0) {
+                    td->sid = (uint32_t)sid; td->c0 = c0; td->nc = nc; td->g0 = g0;
+                    td->lo = lo; td->npos = npos; td->n_in = n_in;
+                }
+            }
+            // distinct output rows over [centres..., negatives...]
+            {
+                const uint32_t nl = nc + nneg;
+                const bool valid = lane < nl;
+                uint32_t t = 0xFFFFFFFFu;
+                if (valid) t = lane < nc ? stok[c0 + lane] : sneg[lane - nc];
+                uint32_t first = lane;
+                if (valid)
+                    for (uint32_t q = 0; q < lane; ++q) {
+                        const uint32_t tq = q < nc ? stok[c0 + q] : sneg[q - nc];
+                        if (tq == t) { first = q; break; }
+                    }
+                const bool isf = valid && first == lane;
+                const uint64_t bal = __ballot(isf);
+                const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
+                const uint32_t my = __shfl(idx, (int)first);
+                const uint32_t n_out = (uint32_t)__popcll(bal);
+                // output row of negative q sits on lane nc + q
+                const uint32_t my_neg = __shfl(my, (int)((nc + lane) & 63));
+                if (lane < OMAX) {
+                    if (isf) td->rout_tok[idx] = t;
+                    if (lane >= n_out) td->rout_tok[lane] = 0xFFFFFFFFu;
+                    td->cen_rout[lane] = (valid && lane < nc) ? (uint8_t)my : 0;
+                    td->neg_rout[lane] = lane < nneg ? (uint8_t)my_neg : 0;
+                    td->bshrink[lane] = lane < nc
+                        ? (uint8_t)(draw(a.seed, c0 + lane, DOM_WINDOW, (uint32_t)sid, a.round).x % w) : 0;
+                }
+                if (lane == 0) td->n_out = n_out;
+            }
+            g0 += ng;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // my_words and my_kept are wave-uniform: lane 0 reports them
+    if (lane == 0) {
+        atomicAdd(&a.counters[C_WORDS], my_words);
+        atomicAdd(&a.counters[C_KEPT], my_kept);
+    }
+}
+
+}  // namespace
+}  // namespace sgns
+
+uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative) {
+    uint32_t b = batch == 0 ? 2 * window + 1 : batch;
+    if (b + negative > (uint32_t)sgns::OMAX) b = sgns::OMAX - negative;
+    if (b + 2 * window > (uint32_t)sgns::RMAX) b = sgns::RMAX - 2 * window;
+    return b;
+}
+
+uint32_t sgns_tile_rows(uint32_t centres, uint32_t window) {
+    return centres + 2 * window <= 48 ? 48u : 64u;
+}
+
+#define HIP_TRY(x)                              \
+    do {                                        \
+        hipError_t e_ = (x);                    \
+        if (e_ != hipSuccess) return e_;        \
+    } while (0)
+
+hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
+                    uint32_t round, float alpha, const SgnsDebug *dbg, float *ms) {
+    using namespace sgns;
+    Params a;
+    a.off = d_off;
+    a.tok = d_tok;
+    a.n_sent = n_sent;
+    a.syn0 = c->d_syn0;
+    a.syn1 = c->d_syn1;
+    a.dp = c->dp;
+    a.d = c->d;
+    a.in_vocab = c->d_in_vocab;
+    a.vocab_rank = c->d_vocab_rank;
+    a.keep_thr = c->d_keep_thr;
+    a.uni_thr = c->d_uni_thr;
+    a.uni_alias = c->d_uni_alias;
+    a.V = c->V;
+    a.window = c->opt.window;
+    a.negative = c->opt.negative;
+    a.bgroup = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
+    a.rmax = sgns_tile_rows(a.bgroup, c->opt.window);
+    a.compat_break = c->opt.compat_vocab_break;
+    const int mode = c->opt.update_mode;
+    a.writeback = mode == FELOG_UPDATE_WRITEBACK || (mode == FELOG_UPDATE_DEFAULT && c->opt.precision == FELOG_BF16);
+    a.alpha = alpha;
+    a.round = round;
+    a.seed = c->opt.seed;
+    a.counters = (unsigned long long *)c->d_counters;
+    a.dbg_max = dbg ? dbg->max_groups : 0;
+    a.dbg_groups = dbg ? dbg->groups : nullptr;
+    a.dbg_rows = dbg ? dbg->rows : nullptr;
+    a.dbg_logits = dbg ? dbg->logits : nullptr;
+    a.dbg_dv = dbg ? dbg->dv : nullptr;
+    a.dbg_du = dbg ? dbg->du : nullptr;
+    // every tile holds at least one centre and every sentence ends one tile
+    const uint64_t need = n_tok / a.bgroup + n_sent + 64;
+    if (need > c->tiles_cap) {
+        dfree(c->d_tiles);
+        c->d_tiles = nullptr;
+        c->tiles_cap = 0;
+        const uint64_t cap = need + need / 4;
+        HIP_TRY(hipMalloc(&c->d_tiles, cap * kDescBytes));
+        c->tiles_cap = cap;
+    }
+    a.tiles = (unsigned char *)c->d_tiles;
+    a.tiles_cap = c->tiles_cap;
+
+    HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_DEBUG - C_WORDS + 1), c->stream));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (n_sent > 0) {
+        uint64_t pblocks = (n_sent + kPrepWaves - 1) / kPrepWaves;
+        const uint64_t pcap = (uint64_t)c->cu_count * 16;
+        if (pblocks > pcap) pblocks = pcap;
+        k_sgns_prep<<<(unsigned)pblocks, kBlock, 0, c->stream>>>(a);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    if (n_sent > 0) HIP_TRY(launch_tile(c, a));
+    HIP_TRY(hipEventRecord(c->ev2, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev2));
+    HIP_TRY(hipEventElapsedTime(&ms[0], c->ev0, c->ev2));
+    HIP_TRY(hipEventElapsedTime(&ms[1], c->ev0, c->ev1));
+    HIP_TRY(hipEventElapsedTime(&ms[2], c->ev1, c->ev2));
+    HIP_TRY(hipMemcpy(c->h_counters, c->d_counters, kNumCounters * 8, hipMemcpyDeviceToHost));
+    return hipSuccess;
+}
+
+}  // namespace felog

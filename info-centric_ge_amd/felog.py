@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libfelog_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "felog.h")
 
 FP32, BF16 = 0, 1
+UPDATE_DEFAULT, UPDATE_ATOMIC, UPDATE_WRITEBACK = 0, 1, 2
 STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_IO", -3: "ERR_HIP", -4: "ERR_OOM",
           -5: "ERR_INVARIANT", -6: "ERR_STATE", -7: "ERR_CAPACITY"}
 
@@ -41,7 +42,7 @@ class Options(C.Structure):
         ("min_count", C.c_uint32), ("binary", C.c_int32), ("precision", C.c_int32),
         ("max_walk_len", C.c_uint32), ("overflow_capacity", C.c_uint64), ("seed", C.c_uint64),
         ("compat_lr", C.c_int32), ("compat_vocab_break", C.c_int32), ("max_rounds", C.c_uint32),
-        ("device", C.c_int32),
+        ("device", C.c_int32), ("update_mode", C.c_int32),
     ]
 
 
@@ -60,6 +61,7 @@ class Stats(C.Structure):
         ("build_ms", C.c_double), ("walk_kernel_ms", C.c_double), ("compact_kernel_ms", C.c_double),
         ("train_kernel_ms", C.c_double), ("eval_kernel_ms", C.c_double),
         ("walk_ms_total", C.c_double), ("train_ms_total", C.c_double), ("eval_ms_total", C.c_double),
+        ("sgns_prep_ms", C.c_double), ("sgns_tile_ms", C.c_double),
     ]
 
 
@@ -289,5 +291,5 @@ class Felog:
         self._check(self.lib.felog_debug_sgns(self.ctx, _ptr(offsets), _ptr(tokens), len(offsets) - 1, rnd,
                                               alpha, G, C.byref(n), _ptr(groups), _ptr(rows), _ptr(logits),
                                               _ptr(dv), _ptr(du)), "felog_debug_sgns")
-        k = n.value
-        return {"groups": groups[:k], "rows": rows[:k], "logits": logits[:k], "dv": dv[:k], "du": du[:k]}
+        live = np.nonzero(groups[: n.value, 2] > 0)[0]  # slots of sentences without a tile stay empty
+        return {"groups": groups[live], "rows": rows[live], "logits": logits[live], "dv": dv[live], "du": du[live]}

@@ -390,7 +390,7 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
         if (m < 2) continue;
         for (uint32_t g0 = 0; g0 * B < m;) {
             /* Tile: consecutive groups g0.. while centres + k per group <= 32
-             * output slots and the windows span <= 64 positions. */
+             * output slots and the windows span <= rmax positions. */
             const uint32_t c0 = g0 * B;
             const uint32_t lo = c0 >= w ? c0 - w : 0;
             uint32_t ng = 1;
@@ -399,7 +399,7 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
                 if (gn * B >= m) break;
                 const uint32_t ce = (gn + 1) * B < m ? (gn + 1) * B : m;
                 const uint32_t he = ce + w < m ? ce + w : m;
-                if ((ce - c0) + k * (ng + 1) > OMAX || he - lo > RMAX) break;
+                if ((ce - c0) + k * (ng + 1) > OMAX || he - lo > p->rmax) break;
                 ++ng;
             }
             const uint32_t c_end = (g0 + ng) * B < m ? (g0 + ng) * B : m;

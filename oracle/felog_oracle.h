@@ -78,7 +78,7 @@ void orc_init_syn0(uint32_t V, uint32_t d, const uint32_t *rank, float *syn0);
  * applied in sentence order to syn0/syn1 (FP64 copies), no buffers.
  * Returns the number of groups. */
 typedef struct orc_sgns_params {
-    uint32_t V, d, window, negative, bgroup;
+    uint32_t V, d, window, negative, bgroup, rmax;
     int compat_break;
     float alpha;
     uint32_t round;

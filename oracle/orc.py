@@ -158,7 +158,8 @@ def init_syn0(V, d, rank):
 
 class SgnsParams(C.Structure):
     _fields_ = [("V", C.c_uint32), ("d", C.c_uint32), ("window", C.c_uint32), ("negative", C.c_uint32),
-                ("bgroup", C.c_uint32), ("compat_break", C.c_int), ("alpha", C.c_float), ("round", C.c_uint32),
+                ("bgroup", C.c_uint32), ("rmax", C.c_uint32), ("compat_break", C.c_int), ("alpha", C.c_float),
+                ("round", C.c_uint32),
                 ("seed", C.c_uint64), ("in_vocab", C.c_void_p), ("vocab_rank", C.c_void_p),
                 ("keep_thr", C.c_void_p), ("uni_thr", C.c_void_p), ("uni_alias", C.c_void_p)]
 
@@ -170,14 +171,20 @@ def group_centres(batch, window, negative):
     return b
 
 
+def tile_rows(centres, window):
+    """Input positions a tile may span: 48 when one group's windows fit, else 64."""
+    return 48 if centres + 2 * window <= 48 else 64
+
+
 def sgns_groups(tables, offsets, tokens, syn0, syn1, *, d, window, negative, batch, alpha, rnd, seed,
                 compat_break=False, mode=0, max_groups=4096):
     """FP64 SGNS oracle.  tables = dict(in_vocab, vocab_rank, keep_thr, uni_thr, uni_alias)."""
     keep = [np.ascontiguousarray(tables[k], np.uint8 if k == "in_vocab" else np.uint32)
             for k in ("in_vocab", "vocab_rank", "keep_thr", "uni_thr", "uni_alias")]
     V = len(keep[0])
-    p = SgnsParams(V, d, window, negative, group_centres(batch, window, negative), int(compat_break), alpha, rnd,
-                   seed, *[k.ctypes.data for k in keep])
+    B = group_centres(batch, window, negative)
+    p = SgnsParams(V, d, window, negative, B, tile_rows(B, window), int(compat_break), alpha, rnd, seed,
+                   *[k.ctypes.data for k in keep])
     s0 = np.ascontiguousarray(syn0, np.float64).copy()
     s1 = np.ascontiguousarray(syn1, np.float64).copy()
     offsets = np.ascontiguousarray(offsets, np.uint64)
