@@ -56,6 +56,19 @@ def sgns_bytes_per_word(d, k, batch, esize):
     return 2 * d * esize + 2 * d * esize + 2 * k * d * esize / batch + 4
 
 
+def measured_traffic(args):
+    """HBM bytes per launch from the committed rocprofv3 counter summary of
+    this workload (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy
+    calibration), or {} when there is none for it."""
+    tag = {"rmat-1.1M": "rmat1.1M"}.get(args.workload)
+    path = os.path.join(ROOT, "profiles", f"traffic_{tag}_{'f32' if args.precision == 'fp32' else 'bf16'}.json")
+    if tag is None or args.dim != 128 or not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        kernels = json.load(fh)["kernels"]
+    return {name: round(v["hbm_bytes"]) for name, v in kernels.items()}
+
+
 def cpu_baseline(src, dst, w, n, args, seed, threads):
     """The oracle (CPU restatement) timed on the host cores over a bounded
     sample of walkers: walks, then word2vec skip-gram on their corpus."""
@@ -146,19 +159,21 @@ def main():
 
     K = args.steps
     walk_gbs = steps * 24 / (walk_ms / 1e3) / 1e9
-    sgns_gbs = kept * sgns_bytes_per_word(args.dim, 5, 21, esize) / (train_ms / 1e3) / 1e9
-    sgns_tfs = pairs * 6 * 6 * args.dim / (train_ms / 1e3) / 1e12
-    if train_ms >= walk_ms:
-        if sgns_tfs / FP32_PEAK_TFS > sgns_gbs / HBM_PEAK_GBS and prec == felog.FP32:
-            roof = {"kernel": "k_sgns", "bound": "mfma", "achieved": sgns_tfs, "peak": FP32_PEAK_TFS,
-                    "unit": "TFLOP/s", "frac": sgns_tfs / FP32_PEAK_TFS}
-        else:
-            roof = {"kernel": "k_sgns", "bound": "hbm", "achieved": sgns_gbs, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": sgns_gbs / HBM_PEAK_GBS}
+    # the tile kernel moves the rows; the prep kernel reads the 4-byte tokens
+    tile_bytes_word = sgns_bytes_per_word(args.dim, 5, 21, esize) - 4
+    sgns_gbs = kept * tile_bytes_word / (tile_ms / 1e3) / 1e9
+    sgns_tfs = pairs * 6 * 6 * args.dim / (tile_ms / 1e3) / 1e12
+    traffic = measured_traffic(args)
+    if tile_ms >= walk_ms:
+        roof = {"kernel": "k_sgns_tile", "bound": "hbm", "achieved": sgns_gbs, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": sgns_gbs / HBM_PEAK_GBS,
+                "traffic": traffic.get("k_sgns_tile"),
+                "algorithmic_bytes_per_launch": kept / K * tile_bytes_word,
+                "avg_launch_ms": tile_ms / K}
     else:
         roof = {"kernel": "k_walk", "bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": walk_gbs / HBM_PEAK_GBS}
-    roof["traffic"] = None
+                "frac": walk_gbs / HBM_PEAK_GBS, "traffic": traffic.get("k_walk"),
+                "algorithmic_bytes_per_launch": steps / K * 24, "avg_launch_ms": walk_ms / K}
     out = {
         "metric": METRIC,
         "value": tok / elapsed,
