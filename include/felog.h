@@ -91,13 +91,15 @@ typedef struct felog_options {
 } felog_options;
 
 /* How SGNS tiles write their row deltas (Hogwild either way).
+ * WRITEBACK (the default): each tile stores staged-row + delta (bf16:
+ *   stochastically rounded), like word2vec's unsynchronised CPU threads; a
+ *   concurrent tile's delta to the same row can be overwritten, which bounds
+ *   the step a hub row takes.
  * ATOMIC: f32 tables get hardware float atomic adds, so no delta is lost
  *   (as src/word2vec.cu:530,533); bf16 tables a compare-and-swap per pair of
- *   elements with stochastic rounding.
- * WRITEBACK: each tile stores staged-row + delta (bf16: stochastically
- *   rounded), like word2vec's unsynchronised CPU threads; a concurrent
- *   tile's delta to the same row can be overwritten.
- * DEFAULT: ATOMIC for f32 tables, WRITEBACK for bf16 tables. */
+ *   elements with stochastic rounding.  Every concurrent tile's delta, each
+ *   computed from the same stale row, lands: on degree-skewed graphs hub rows
+ *   then take hundreds of steps at once and training diverges. */
 enum { FELOG_UPDATE_DEFAULT = 0, FELOG_UPDATE_ATOMIC = 1, FELOG_UPDATE_WRITEBACK = 2 };
 
 typedef struct felog_ctx felog_ctx;

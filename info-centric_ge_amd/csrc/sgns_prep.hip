@@ -219,7 +219,7 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
     a.rmax = sgns_tile_rows(a.bgroup, c->opt.window);
     a.compat_break = c->opt.compat_vocab_break;
     const int mode = c->opt.update_mode;
-    a.writeback = mode == FELOG_UPDATE_WRITEBACK || (mode == FELOG_UPDATE_DEFAULT && c->opt.precision == FELOG_BF16);
+    a.writeback = mode != FELOG_UPDATE_ATOMIC;
     a.alpha = alpha;
     a.round = round;
     a.seed = c->opt.seed;
