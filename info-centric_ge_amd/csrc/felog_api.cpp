@@ -57,7 +57,7 @@ float round_alpha(const felog_options &o, uint32_t round) {
 void free_all(felog_ctx *c) {
     void *ptrs[] = {c->d_esrc, c->d_edst, c->d_ew, c->d_row_ptr, c->d_col, c->d_w, c->d_vrec, c->d_erec,
                     c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni_thr,
-                    c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_syn1, c->d_flags,
+                    c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
                     c->d_scores, c->d_active, c->d_walk_ok, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp};
     for (void *p : ptrs) dfree(p);

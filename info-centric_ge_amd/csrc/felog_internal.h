@@ -68,7 +68,8 @@ struct felog_ctx {
     uint32_t *d_keep_thr = nullptr;      // V
     double *d_T = nullptr, *d_dT = nullptr;  // n log2 n table and its differences
 
-    // embedding tables (f32 or bf16 rows of dp elements, indexed by vertex id)
+    // embedding tables (f32 or bf16 rows of dp elements, indexed by vertex
+    // id); one allocation owned by d_syn0, d_syn1 = d_syn0 + V rows
     void *d_syn0 = nullptr, *d_syn1 = nullptr;
 
     // convergence state

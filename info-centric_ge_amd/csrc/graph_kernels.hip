@@ -429,8 +429,12 @@ hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg) {
 hipError_t init_tables(felog_ctx *c) {
     const uint64_t n = (uint64_t)c->V * c->dp;
     const size_t es = c->opt.precision == FELOG_BF16 ? 2 : 4;
-    if (!c->d_syn0) HIP_TRY(hipMalloc(&c->d_syn0, n * es));
-    if (!c->d_syn1) HIP_TRY(hipMalloc(&c->d_syn1, n * es));
+    // one allocation: syn1neg directly follows syn0, so a row of either
+    // table is addressed from syn0 by one row index (syn1neg row v = V + v)
+    if (!c->d_syn0) {
+        HIP_TRY(hipMalloc(&c->d_syn0, 2 * n * es));
+        c->d_syn1 = (unsigned char *)c->d_syn0 + n * es;
+    }
     Affine jump[64];
     unsigned long long a = 25214903917ull, cc = 11ull;
     for (int k = 0; k < 64; ++k) {

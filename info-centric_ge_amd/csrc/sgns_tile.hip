@@ -15,11 +15,18 @@
 // taken before the tile's writes, i.e. HogBatch's corrM / dOut / dIn
 // (include/dsgl.hpp:1241-1310) over deduplicated rows.
 //
-// Tiles are assigned round-robin to workgroups; each workgroup keeps the
-// next tile's descriptor in flight in registers while it computes.  Latency
-// of the row gathers is hidden by running several workgroups per CU (the
-// LDS footprint is sized for three at d = 128 in f32).
+// Work per tile and workgroup:
+//   1. descriptor into LDS, count tile cleared;
+//   2. row gathers issued into registers, pair counts built with LDS
+//      atomics while they are in flight, gathered rows stored to LDS;
+//   3. X and G (G^T overwrites the counts in place);
+//   4. dVin and dUout, each 16x16 output tile written straight to the table.
+// Every loop index is advanced incrementally (no integer division in the
+// tile loop) and each lane's table row addresses are formed once per 16-row
+// block.  Latency of the gathers is hidden by several workgroups per CU (the
+// LDS footprint allows three at d = 128 in f32).
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include "felog_dev.h"
 #include "felog_internal.h"
 #include "sgns_common.h"
@@ -32,30 +39,82 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
+static_assert(offsetof(TileDesc, rout_tok) == offsetof(TileDesc, rin_tok) + 4 * RMAX,
+              "output row ids follow the input row ids");
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s4 lds_s4;
+typedef __attribute__((address_space(3))) unsigned char lds_u8;
+
+// Pitch (words) of the OMAX-row count / G^T tile.  bf16 tables contract
+// over 32 input rows per MFMA, so the rows are padded to a multiple of 32.
+__host__ __device__ constexpr uint32_t gt_pitch(uint32_t esize, uint32_t rmax) {
+    return esize == 4 ? rmax + 4 : (rmax + 31) / 32 * 32 + 4;
+}
+
+// LDS: descriptor | RM input rows then OMAX output rows (pitch ldv) | the
+// OMAX x gt_pitch count / G^T tile.
 struct Lds {
     uint32_t ldv;  // row pitch of staged rows, in elements
-    size_t o_desc, o_v, o_u, o_g, total;
+    size_t o_desc, o_rows, o_g, total;
 };
 
 __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax) {
     Lds L;
-    L.ldv = dp + (esize == 4 ? 4u : 8u);  // 16-byte skew between rows
+    L.ldv = dp + 16 / esize;  // 16-byte skew between rows
     size_t o = 0;
     L.o_desc = o; o += kDescBytes;
-    L.o_v = o;    o += (size_t)rmax * L.ldv * esize;
-    L.o_u = o;    o += (size_t)OMAX * L.ldv * esize;
-    L.o_g = o;    o += (size_t)rmax * GLD * 4;
+    L.o_rows = o; o += (size_t)(rmax + OMAX) * L.ldv * esize;
+    L.o_g = o;    o += (size_t)OMAX * gt_pitch(esize, rmax) * 4;
     L.total = o;
     return L;
+}
+
+// ds_read_b64_tr_b16: within each 16-lane group, lane 4j+q supplies the
+// address of four consecutive 16-bit elements (chunk C[4j+q]) and lane i
+// receives element i%4 of chunks C[i/4], C[4+i/4], C[8+i/4], C[12+i/4];
+// i.e. the group reads a 4 x 16 block whose row j is C[4j..4j+3] and lane i
+// gets its column i.  Pointing the lanes of row j at row k_j of a row-major
+// matrix (columns n0..n0+15) hands lane i the four values M[k_j][n0+i].
+__device__ __forceinline__ s4 tr_load(lds_u8 *base, uint32_t byte_off) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s4 *)(base + byte_off));
+}
+
+__device__ __forceinline__ bf8 join8(s4 a, s4 b) {
+    return __builtin_bit_cast(bf8, __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7));
 }
 
 __device__ __forceinline__ float load_elem(const float *p) { return *p; }
 __device__ __forceinline__ float load_elem(const uint16_t *p) { return bf16_to_f32(*p); }
 
+// sigma(x) with word2vec's clamp (0 below -MAX_EXP, 1 above), branch-free
 __device__ __forceinline__ float sigmoid_clamped(float x) {
-    if (x > kMaxExp) return 1.f;
-    if (x < -kMaxExp) return 0.f;
-    return 1.f / (1.f + __expf(-x));
+    const float s = __builtin_amdgcn_rcpf(1.f + __expf(-x));
+    return x > kMaxExp ? 1.f : (x < -kMaxExp ? 0.f : s);
+}
+
+// alpha (P - (P+Q) sigma(x)) for a packed count word (P low, Q high half)
+__device__ __forceinline__ float pair_grad(uint32_t cc, float x, float alpha) {
+    const float P = (float)(cc & 0xFFFFu);
+    const float PQ = (float)((cc & 0xFFFFu) + (cc >> 16));
+    return alpha * (P - PQ * sigmoid_clamped(x));
+}
+
+// value of the adjacent lane (l ^ 1), by DPP quad permutation [1,0,3,2]
+__device__ __forceinline__ float swap_pair(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+// 32 rounding bits: murmur3's finaliser over a Weyl sequence seeded by one
+// Philox draw per lane and tile
+__device__ __forceinline__ uint32_t round_bits(uint32_t base, uint32_t n) {
+    uint32_t h = base + n * 0x9E3779B9u;
+    h ^= h >> 16; h *= 0x85EBCA6Bu;
+    h ^= h >> 13; h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
 }
 
 // bf16 pair += (d0, d1) with stochastic rounding, by compare-and-swap on the
@@ -82,63 +141,105 @@ __device__ __forceinline__ void bf16_pair_store_sr(uint16_t *p, const uint16_t *
         (uint32_t)f32_to_bf16_sr(lo, rnd) | ((uint32_t)f32_to_bf16_sr(hi, rnd >> 16) << 16);
 }
 
-// Write the 16x16 delta tile held in acc (MFMA output layout: lane l holds
-// rows 4*(l/16)+0..3 of column l%16) to rows tok[row0 + ...] of the table.
+// Table rows one lane writes for a 16-row block (MFMA output layout: lane l
+// holds rows r0..r0+3, r0 = 16*rt + 4*(l/16), of one column): all four in
+// f32; in bf16 adjacent columns pair up across lanes l, l^1 into one 32-bit
+// store, even lanes writing rows 0-1 of the four and odd lanes rows 2-3.
+template <typename T>
+struct LaneRows {
+    static constexpr int N = sizeof(T) == 4 ? 4 : 2;
+    T *p[N];          // table row
+    uint32_t rel[N];  // row within the lane's four
+    bool ok[N];       // row exists in the tile
+};
+
+template <typename T>
+__device__ __forceinline__ LaneRows<T> lane_rows(T *table, const uint32_t *ids, uint32_t r0, uint32_t n,
+                                                 uint32_t dp, uint32_t lane) {
+    LaneRows<T> R;
+#pragma unroll
+    for (int h = 0; h < LaneRows<T>::N; ++h) {
+        const uint32_t j = sizeof(T) == 4 ? (uint32_t)h : ((lane & 1) ? 2u + h : (uint32_t)h);
+        R.rel[h] = j;
+        R.ok[h] = r0 + j < n;
+        R.p[h] = table + (uint64_t)ids[R.ok[h] ? r0 + j : 0] * dp;
+    }
+    return R;
+}
+
+// Write one 16x16 output tile held in acc to the lane's rows.  f32
+// write-back: acc already holds the staged value plus the delta; f32 atomic:
+// acc is the delta.  bf16: staged pair + delta pair, stochastically rounded
+// (write-back), or added by compare-and-swap (atomic).
 template <typename T, bool WB>
-__device__ __forceinline__ void apply_tile(const Params &a, T *table, const T *staged, uint32_t ldv,
-                                           const uint32_t *tok, uint32_t nrows, uint32_t row0, uint32_t col,
-                                           uint32_t lane, const f4 &acc, uint32_t rnd_x, uint32_t g0,
-                                           uint32_t sid) {
-    const uint32_t dp = a.dp;
-    const uint32_t grp = lane >> 4;
+__device__ __forceinline__ void store_tile(const LaneRows<T> &R, const T *staged, uint32_t ldv, uint32_t col,
+                                           uint32_t d, uint32_t lane, const f4 &acc, uint32_t rbase,
+                                           uint32_t &rn) {
     if constexpr (sizeof(T) == 4) {
-        if (col >= a.d) return;
-        for (int jj = 0; jj < 4; ++jj) {
-            const uint32_t r = row0 + 4 * grp + jj;
-            if (r >= nrows) continue;
-            float *dst = (float *)table + (uint64_t)tok[r] * dp + col;
-            if constexpr (WB) *dst = ((const float *)staged)[r * ldv + col] + acc[jj];
-            else unsafeAtomicAdd(dst, acc[jj]);
+        if (col >= d) return;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (!R.ok[j]) continue;
+            if constexpr (WB) R.p[j][col] = acc[j];
+            else unsafeAtomicAdd(&R.p[j][col], acc[j]);
         }
     } else {
-        // pair adjacent columns across lanes l, l^1: even lanes write rows
-        // 0-1 of their 4, odd lanes rows 2-3
         float other[4];
-        for (int jj = 0; jj < 4; ++jj) other[jj] = __shfl_xor(acc[jj], 1);
-        const u32x4 rnd = draw(a.seed, rnd_x, DOM_ROUND_BF16 | g0, sid, a.round);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) other[j] = swap_pair(acc[j]);
         const bool odd = lane & 1;
         const uint32_t c2 = col & ~1u;
+#pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int jj = odd ? 2 + h : h;
-            const uint32_t r = row0 + 4 * grp + jj;
-            const float lo_v = odd ? other[jj] : acc[jj];
-            const float hi_v = odd ? acc[jj] : other[jj];
-            if (r >= nrows || c2 >= a.d) continue;
-            uint16_t *dst = (uint16_t *)table + (uint64_t)tok[r] * dp + c2;
-            const uint32_t bits = h == 0 ? rnd.x : rnd.y;
-            if constexpr (WB) bf16_pair_store_sr(dst, (const uint16_t *)staged + r * ldv + c2, lo_v, hi_v, bits);
-            else bf16_pair_add_sr(dst, lo_v, hi_v, bits);
+            const uint32_t bits = round_bits(rbase, rn++);
+            const float lo_v = odd ? other[2 + h] : acc[h];
+            const float hi_v = odd ? acc[2 + h] : other[h];
+            if (!R.ok[h] || c2 >= d) continue;
+            if constexpr (WB)
+                bf16_pair_store_sr(R.p[h] + c2, staged + R.rel[h] * ldv + c2, lo_v, hi_v, bits);
+            else
+                bf16_pair_add_sr(R.p[h] + c2, lo_v, hi_v, bits);
         }
     }
 }
 
 template <typename T, int RM, bool WB>
-__global__ __launch_bounds__(kBlock) void k_sgns_tile(Params a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 4))) void k_sgns_tile(Params a) {
+    constexpr uint32_t GT = gt_pitch(sizeof(T), RM);  // pitch of the count / G^T tile (words)
+    constexpr uint32_t EPV = 16 / sizeof(T);          // elements per 16-byte vector
+    constexpr int NB = sizeof(T) == 4 ? 8 : 5;        // gathered vectors held per thread
     extern __shared__ __align__(16) unsigned char lds[];
     const Lds L = lds_layout(a.dp, sizeof(T), RM);
     TileDesc &td = *(TileDesc *)(lds + L.o_desc);
-    T *Vs = (T *)(lds + L.o_v);
-    T *Us = (T *)(lds + L.o_u);
-    float *Gs = (float *)(lds + L.o_g);
-    uint32_t *cnt = (uint32_t *)(lds + L.o_g);  // counts share the G tile
+    const uint32_t *toks = (const uint32_t *)(lds + L.o_desc + offsetof(TileDesc, rin_tok));
+    const uint32_t ldv = L.ldv;
+    T *Vs = (T *)(lds + L.o_rows);
+    T *Us = Vs + RM * ldv;
+    float *Gt = (float *)(lds + L.o_g);
+    uint32_t *cnt = (uint32_t *)Gt;  // counts, overwritten in place by G^T
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t l16 = lane & 15, g4 = 4 * (lane >> 4);
     const uint32_t w = a.window, k = a.negative, B = a.bgroup;
-    const uint32_t dp = a.dp, ldv = L.ldv;
-    const uint32_t row16 = lane & 15, grp = lane >> 4;
+    const uint32_t dp = a.dp, d = a.d, dtiles = dp >> 4;
     T *syn0 = (T *)a.syn0;
     T *syn1 = (T *)a.syn1;
     unsigned long long my_pairs = 0, my_tiles = 0;
+
+    // gathers: vector v = tid + kBlock*j of a tile covers row v / vpr and
+    // 16-byte column v % vpr; (row, column) advance by fixed steps
+    const uint32_t vpr = dp / EPV;
+    const uint32_t st_r0 = tid / vpr, st_c0 = tid % vpr;
+    const uint32_t st_dr = kBlock / vpr, st_dc = kBlock % vpr;
+    // pair counts: eight threads per centre, strided over its window
+    const uint32_t ci = tid >> 3, cs = tid & 7, span = 2 * w + 1;
+    // f32 write-back tiles start from the staged rows (acc = old + delta)
+    const bool acc_old = WB && sizeof(T) == 4 && a.dbg_max == 0;
+
+    // bf16 products contract over whole 32-row blocks, reading staged rows
+    // past a tile's own (rows of earlier tiles, finite); clear them once
+    if constexpr (sizeof(T) == 2)
+        for (uint32_t i = tid; i < (RM + OMAX) * ldv * 2 / 16; i += kBlock) ((uint4 *)Vs)[i] = make_uint4(0, 0, 0, 0);
 
     uint64_t n_tiles = a.n_sent + a.counters[C_TILES];
     if (n_tiles > a.tiles_cap) n_tiles = a.tiles_cap;
@@ -149,57 +250,93 @@ __global__ __launch_bounds__(kBlock) void k_sgns_tile(Params a) {
 
     for (; tile < n_tiles; tile += gridDim.x) {
         if (tid < (uint32_t)kDescVec) ((uint4 *)&td)[tid] = pre;
-        for (uint32_t i = tid; i < (uint32_t)(RM * GLD); i += kBlock) cnt[i] = 0;
+        for (uint32_t i = tid; i < OMAX * GT / 4; i += kBlock) ((uint4 *)cnt)[i] = make_uint4(0, 0, 0, 0);
         __syncthreads();
-        // the next descriptor is consumed one tile later
-        const uint64_t nxt = tile + gridDim.x;
-        if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
 
         const uint32_t c0 = td.c0, nc = td.nc, g0 = td.g0, lo = td.lo, npos = td.npos;
         const uint32_t n_in = td.n_in, n_out = td.n_out;
+        const uint64_t nxt = tile + gridDim.x;
         if (nc == 0) {  // sentence without a tile
+            if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
             __syncthreads();
             continue;
         }
         const uint32_t R16 = (n_in + 15) & ~15u, O16 = (n_out + 15) & ~15u;
+        const uint32_t nrows = R16 + O16;
 
-        // pair counts: P in the low half-word, Q in the high half-word
-        const uint32_t span = 2 * w + 1;
-        for (uint32_t q = tid; q < nc * span; q += kBlock) {
-            const uint32_t i = q / span, aa = q % span;
-            const uint32_t b = td.bshrink[i];
-            if (aa == w || aa < b || aa >= span - b) continue;
-            const int p = (int)(c0 + i) - (int)w + (int)aa - (int)lo;  // input position in the tile
-            if (p < 0 || p >= (int)npos) continue;
-            const uint32_t rin = td.pos_rin[p];
-            const uint32_t ro = td.cen_rout[i];
-            atomicAdd(&cnt[rin * GLD + ro], 1u);
-            const uint32_t nb = ((c0 + i) / B - g0) * k;  // negatives of this centre's group
-            for (uint32_t j = 0; j < k; ++j) {
-                const uint32_t no = td.neg_rout[nb + j];
-                if (no != ro) atomicAdd(&cnt[rin * GLD + no], 1u << 16);
-            }
-            ++my_pairs;
+        // ---- 1. gathers of this thread's first NB vectors (padding rows 0):
+        //         slots and LDS reads of the row ids first, then the loads ----
+        // Staged row s < R16 is input row s, s >= R16 output row s - R16 (LDS
+        // row RM + s - R16, table row V + id in the joint allocation).
+        auto advance = [&](uint32_t &r, uint32_t &cc) {
+            r += st_dr;
+            cc += st_dc;
+            if (cc >= vpr) { cc -= vpr; ++r; }
+        };
+        uint4 buf[NB];
+        uint32_t tix[NB], ce[NB];
+        uint32_t row = st_r0, c = st_c0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const bool live = row < n_in || (row >= R16 && row < R16 + n_out);
+            tix[j] = live ? (row < R16 ? row : row + (RMAX - R16)) : 0xFFFFFFFFu;
+            ce[j] = c * EPV;
+            advance(row, c);
         }
-        // stage rows (16-byte vectors); padding rows are zero
-        {
-            constexpr uint32_t epv = 16 / sizeof(T);
-            const uint32_t vpr = dp / epv;
-            const uint32_t nv = (R16 + O16) * vpr;
-            for (uint32_t i = tid; i < nv; i += kBlock) {
-                const uint32_t row = i / vpr, c = i % vpr;
-                uint4 val = make_uint4(0, 0, 0, 0);
-                T *dst;
-                if (row < R16) {
-                    if (row < n_in) val = *(const uint4 *)(syn0 + (uint64_t)td.rin_tok[row] * dp + c * epv);
-                    dst = Vs + row * ldv + c * epv;
-                } else {
-                    const uint32_t o = row - R16;
-                    if (o < n_out) val = *(const uint4 *)(syn1 + (uint64_t)td.rout_tok[o] * dp + c * epv);
-                    dst = Us + o * ldv + c * epv;
+        uint32_t grow[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            grow[j] = tix[j] != 0xFFFFFFFFu ? toks[tix[j]] + (tix[j] >= (uint32_t)RMAX ? a.V : 0u) : 0u;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            buf[j] = make_uint4(0, 0, 0, 0);
+            if (tix[j] != 0xFFFFFFFFu) buf[j] = *(const uint4 *)(syn0 + ((uint64_t)grow[j] * dp + ce[j]));
+        }
+        // the next descriptor is consumed one tile later
+        if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
+
+        // ---- 2. pair counts: P in the low half-word, Q in the high -------------
+        if (ci < nc) {
+            const uint32_t b = td.bshrink[ci];
+            const uint32_t ro = td.cen_rout[ci];
+            const uint32_t nb = ((c0 + ci) / B - g0) * k;  // negatives of this centre's group
+            uint32_t neg[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) neg[j] = (uint32_t)j < k ? td.neg_rout[nb + j] : 0u;
+            for (uint32_t aa = cs; aa < span; aa += 8) {
+                if (aa == w || aa < b || aa >= span - b) continue;
+                const int p = (int)(c0 + ci) - (int)w + (int)aa - (int)lo;  // input position in the tile
+                if (p < 0 || p >= (int)npos) continue;
+                const uint32_t rin = td.pos_rin[p];
+                atomicAdd(&cnt[ro * GT + rin], 1u);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if ((uint32_t)j < k && neg[j] != ro) atomicAdd(&cnt[neg[j] * GT + rin], 1u << 16);
+                for (uint32_t j = 8; j < k; ++j) {
+                    const uint32_t no = td.neg_rout[nb + j];
+                    if (no != ro) atomicAdd(&cnt[no * GT + rin], 1u << 16);
                 }
-                *(uint4 *)dst = val;
+                ++my_pairs;
             }
+        }
+
+        // ---- 3. gathered vectors to LDS; any further vectors directly ---------
+        {
+            uint32_t r = st_r0, cc = st_c0;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (r < nrows) *(uint4 *)(Vs + (r < R16 ? r : r + (RM - R16)) * ldv + cc * EPV) = buf[j];
+                advance(r, cc);
+            }
+        }
+        while (row < nrows) {
+            const bool in = row < R16;
+            const uint32_t r = in ? row : row - R16;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (r < (in ? n_in : n_out))
+                val = *(const uint4 *)((in ? syn0 : syn1) + ((uint64_t)toks[in ? r : RMAX + r] * dp + c * EPV));
+            *(uint4 *)(Vs + (in ? row : RM + r) * ldv + c * EPV) = val;
+            advance(row, c);
         }
         __syncthreads();
 
@@ -215,85 +352,219 @@ __global__ __launch_bounds__(kBlock) void k_sgns_tile(Params a) {
                 rrow[tid] = (tid - RMAX) < n_out ? td.rout_tok[tid - RMAX] : 0xFFFFFFFFu;
         }
 
-        // ---- X = Vin * Uout^T, G = alpha (P - (P+Q) sigma(X)) in place ----
-        const uint32_t tilesX = (R16 / 16) * (O16 / 16);
-        for (uint32_t t = wave; t < tilesX; t += 4) {
-            const uint32_t rt = t / (O16 / 16), ot = t % (O16 / 16);
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
-            if constexpr (sizeof(T) == 4) {
-                const float *va = (const float *)Vs + (rt * 16 + row16) * ldv + 4 * grp;
-                const float *ub = (const float *)Us + (ot * 16 + row16) * ldv + 4 * grp;
-                for (uint32_t kb = 0; kb < dp; kb += 16) {
-                    const f4 x = *(const f4 *)(va + kb);
-                    const f4 y = *(const f4 *)(ub + kb);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[0], y[0], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[1], y[1], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[2], y[2], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[3], y[3], acc, 0, 0, 0);
-                }
-            } else {
-                const uint16_t *va = (const uint16_t *)Vs + (rt * 16 + row16) * ldv + 8 * grp;
-                const uint16_t *ub = (const uint16_t *)Us + (ot * 16 + row16) * ldv + 8 * grp;
-                for (uint32_t kb = 0; kb < dp; kb += 32) {
-                    const bf8 x = *(const bf8 *)(va + kb);
-                    const bf8 y = *(const bf8 *)(ub + kb);
+        // ---- 4. X = Vin * Uout^T; G^T = alpha (P - (P+Q) sigma(X))^T in place ----
+        {
+            const uint32_t two = O16 > 16;
+            const uint32_t nX = (R16 >> 4) << two;
+            for (uint32_t t = wave; t < nX; t += 4) {
+                const uint32_t rt = t >> two, ot = t & two;
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+                if constexpr (sizeof(T) == 4) {
+                    // dp is a multiple of 32: two 16-deep blocks per step, both
+                    // blocks' operands read before their MFMAs
+                    const float *va = (const float *)Vs + (rt * 16 + l16) * ldv + g4;
+                    const float *ub = (const float *)Us + (ot * 16 + l16) * ldv + g4;
+                    for (uint32_t kb = 0; kb < dp; kb += 32) {
+                        const f4 x0 = *(const f4 *)(va + kb), y0 = *(const f4 *)(ub + kb);
+                        const f4 x1 = *(const f4 *)(va + kb + 16), y1 = *(const f4 *)(ub + kb + 16);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x0[i], y0[i], acc, 0, 0, 0);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x1[i], y1[i], acc, 0, 0, 0);
+                    }
+                } else {
+                    const uint16_t *va = (const uint16_t *)Vs + (rt * 16 + l16) * ldv + 2 * g4;
+                    const uint16_t *ub = (const uint16_t *)Us + (ot * 16 + l16) * ldv + 2 * g4;
+                    bf8 x = *(const bf8 *)va, y = *(const bf8 *)ub;
+                    for (uint32_t kb = 32; kb < dp; kb += 32) {
+                        const bf8 xn = *(const bf8 *)(va + kb), yn = *(const bf8 *)(ub + kb);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
+                        x = xn;
+                        y = yn;
+                    }
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, y, acc, 0, 0, 0);
                 }
-            }
-            const uint32_t o = ot * 16 + row16;
-            for (int jj = 0; jj < 4; ++jj) {
-                const uint32_t r = rt * 16 + 4 * grp + jj;
-                const uint32_t cc = cnt[r * GLD + o];
-                const float P = (float)(cc & 0xFFFFu), PQ = (float)((cc & 0xFFFFu) + (cc >> 16));
-                Gs[r * GLD + o] = a.alpha * (P - PQ * sigmoid_clamped(acc[jj]));
-                if (dbg_on) a.dbg_logits[(tile * RMAX + r) * OMAX + o] = acc[jj];
+                const uint32_t o = ot * 16 + l16, r0 = rt * 16 + g4;
+                uint32_t *cp = cnt + o * GT + r0;
+                const uint4 cc = *(const uint4 *)cp;
+                f4 gv;
+                gv[0] = pair_grad(cc.x, acc[0], a.alpha);
+                gv[1] = pair_grad(cc.y, acc[1], a.alpha);
+                gv[2] = pair_grad(cc.z, acc[2], a.alpha);
+                gv[3] = pair_grad(cc.w, acc[3], a.alpha);
+                if constexpr (sizeof(T) == 4) {
+                    *(f4 *)cp = gv;
+                } else {
+                    // G = hi + lo in bf16; each 4-row block of G^T holds its
+                    // four hi values, then its four lo values
+                    uint32_t h[4], l[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        h[j] = f32_to_bf16_rne(gv[j]);
+                        l[j] = f32_to_bf16_rne(gv[j] - bf16_to_f32((uint16_t)h[j]));
+                    }
+                    *(uint4 *)cp = make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, l[0] | l[1] << 16, l[2] | l[3] << 16);
+                }
+                if (dbg_on)
+                    for (int j = 0; j < 4; ++j) a.dbg_logits[(tile * RMAX + r0 + j) * OMAX + o] = acc[j];
             }
         }
         __syncthreads();
 
-        // ---- dVin = G * Uout  (R16 x dp, K = O16) ---------------------------
-        const uint32_t dtiles = dp / 16;
-        const uint32_t tilesV = (R16 / 16) * dtiles;
-        for (uint32_t t = wave; t < tilesV; t += 4) {
-            const uint32_t rt = t / dtiles, dt = t % dtiles;
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (uint32_t kb = 0; kb < O16; kb += 16) {
-                const f4 x = *(const f4 *)(Gs + (rt * 16 + row16) * GLD + kb + 4 * grp);
-                const T *ub = Us + (kb + 4 * grp) * ldv + dt * 16 + row16;
-#pragma unroll
-                for (int i4 = 0; i4 < 4; ++i4)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x[i4], load_elem(ub + i4 * ldv), acc, 0, 0, 0);
+        // rounding bits of this lane for the tile (bf16 tables)
+        uint32_t rbase = 0, rn = 0;
+        if constexpr (sizeof(T) == 2) rbase = draw(a.seed, tid, DOM_ROUND_BF16 | g0, td.sid, a.round).x;
+
+        if constexpr (sizeof(T) == 2) {
+            // ---- 5/6 on bf16 MFMA (16x16x32): G = hi + lo, two products each ----
+            lds_u8 *l3 = (lds_u8 *)lds;
+            const uint32_t grp = lane >> 4, jq = l16 >> 2, qq = l16 & 3;
+            const uint32_t gb = (uint32_t)L.o_g, rb = (uint32_t)L.o_rows, rowb = ldv * 2;
+            // dVin = G * Uout, K = all OMAX output rows (zero G past n_out)
+            for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
+                const uint32_t r0 = rt * 16 + g4;
+                const LaneRows<T> rows = lane_rows(syn0, td.rin_tok, r0, n_in, dp, lane);
+                // A = G[r][o], o = 8*grp + s, read transposed from G^T rows
+                const uint32_t a0 = gb + (8 * grp + jq) * GT * 4 + (rt * 4 + qq) * 16;
+                const bf8 ahi = join8(tr_load(l3, a0), tr_load(l3, a0 + 16 * GT));
+                const bf8 alo = join8(tr_load(l3, a0 + 8), tr_load(l3, a0 + 16 * GT + 8));
+                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                    const uint32_t col = dt * 16 + l16;
+                    // B = Uout[o][col], read transposed from the staged rows
+                    const uint32_t b0 = rb + (RM + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
+                    const bf8 bv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
+                    f4 acc = {0.f, 0.f, 0.f, 0.f};
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bv, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bv, acc, 0, 0, 0);
+                    if (a.dbg_max) {
+                        if (dbg_on && col < d)
+                            for (int j = 0; j < 4; ++j) a.dbg_dv[(tile * RMAX + r0 + j) * d + col] = acc[j];
+                    } else {
+                        store_tile<T, WB>(rows, Vs + r0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                    }
+                }
             }
-            const uint32_t col = dt * 16 + row16;
-            if (a.dbg_max) {
-                if (dbg_on && col < a.d)
-                    for (int jj = 0; jj < 4; ++jj)
-                        a.dbg_dv[(tile * RMAX + rt * 16 + 4 * grp + jj) * a.d + col] = acc[jj];
-            } else {
-                apply_tile<T, WB>(a, syn0, Vs, ldv, td.rin_tok, n_in, rt * 16, col, lane, acc, t * 64 + lane, g0,
-                                  td.sid);
+            // dUout = G^T * Vin, K = R16 rounded up to 32 (zero G past n_in;
+            // input rows past RM are output rows, finite)
+            const uint32_t nk32 = (R16 + 31) >> 5;
+            for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
+                const uint32_t o0 = ot * 16 + g4;
+                const LaneRows<T> rows = lane_rows(syn1, td.rout_tok, o0, n_out, dp, lane);
+                // A = G^T[o][r], r = 32*kb + 8*grp + s: two 4-row blocks
+                bf8 ahi[2], alo[2];
+#pragma unroll
+                for (int kb = 0; kb < 2; ++kb) {
+                    const uint4 *gp = (const uint4 *)(lds + gb + ((ot * 16 + l16) * GT + 32 * kb + 8 * grp) * 4);
+                    const uint4 x = (uint32_t)kb < nk32 ? gp[0] : make_uint4(0, 0, 0, 0);
+                    const uint4 y = (uint32_t)kb < nk32 ? gp[1] : make_uint4(0, 0, 0, 0);
+                    ahi[kb] = __builtin_bit_cast(bf8, u4v{x.x, x.y, y.x, y.y});
+                    alo[kb] = __builtin_bit_cast(bf8, u4v{x.z, x.w, y.z, y.w});
+                }
+                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                    const uint32_t col = dt * 16 + l16;
+                    f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int kb = 0; kb < 2; ++kb) {
+                        if ((uint32_t)kb >= nk32) break;
+                        // B = Vin[r][col], read transposed from the staged rows
+                        const uint32_t b0 = rb + (32 * kb + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
+                        const bf8 bv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[kb], bv, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[kb], bv, acc, 0, 0, 0);
+                    }
+                    if (a.dbg_max) {
+                        if (dbg_on && col < d)
+                            for (int j = 0; j < 4; ++j) a.dbg_du[(tile * OMAX + o0 + j) * d + col] = acc[j];
+                    } else {
+                        store_tile<T, WB>(rows, Us + o0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                    }
+                }
+            }
+            ++my_tiles;
+            __syncthreads();  // the next tile reuses the LDS
+            continue;
+        }
+
+        // ---- 5. dVin = G * Uout  (R16 x dp, K = O16) ----------------------------
+        for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
+            const uint32_t r0 = rt * 16 + g4;
+            const LaneRows<T> rows = lane_rows(syn0, td.rin_tok, r0, n_in, dp, lane);
+            // A operand G[r][k], k = 16*kb' + 4*(lane/16) + i: one column of
+            // G^T per k, the same for every column block of this row block
+            const bool o2 = O16 > 16;
+            float ax[8];
+            {
+                const float *ga = Gt + g4 * GT + rt * 16 + l16;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) ax[i] = ga[i * GT];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) ax[4 + i] = o2 ? ga[(16 + i) * GT] : 0.f;
+            }
+            for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                const uint32_t col = dt * 16 + l16;
+                const T *ub = Us + g4 * ldv + col;
+                float bx[8];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) bx[i] = load_elem(ub + i * ldv);
+                if (o2)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) bx[4 + i] = load_elem(ub + (16 + i) * ldv);
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+                if (acc_old)
+                    for (int j = 0; j < 4; ++j) acc[j] = load_elem(Vs + (r0 + j) * ldv + col);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[i], bx[i], acc, 0, 0, 0);
+                if (o2)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[4 + i], bx[4 + i], acc, 0, 0, 0);
+                if (a.dbg_max) {
+                    if (dbg_on && col < d)
+                        for (int j = 0; j < 4; ++j) a.dbg_dv[(tile * RMAX + r0 + j) * d + col] = acc[j];
+                } else {
+                    store_tile<T, WB>(rows, Vs + r0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                }
             }
         }
-        // ---- dUout = G^T * Vin  (O16 x dp, K = R16) ------------------------
-        const uint32_t tilesU = (O16 / 16) * dtiles;
-        for (uint32_t t = wave; t < tilesU; t += 4) {
-            const uint32_t ot = t / dtiles, dt = t % dtiles;
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (uint32_t kb = 0; kb < R16; kb += 16) {
-                const float *gq = Gs + (kb + 4 * grp) * GLD + ot * 16 + row16;
-                const T *vb = Vs + (kb + 4 * grp) * ldv + dt * 16 + row16;
+        // ---- 6. dUout = G^T * Vin  (O16 x dp, K = R16) --------------------------
+        for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
+            const uint32_t o0 = ot * 16 + g4;
+            const LaneRows<T> rows = lane_rows(syn1, td.rout_tok, o0, n_out, dp, lane);
+            // A operand G^T[o][k]: four consecutive words per 16-deep block,
+            // the same for every column block of this row block
+            constexpr int KB = RM / 16;
+            const uint32_t nkb = R16 >> 4;
+            f4 ax[KB];
+            {
+                const float *ga = Gt + (ot * 16 + l16) * GT + g4;
 #pragma unroll
-                for (int i4 = 0; i4 < 4; ++i4)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(gq[i4 * GLD], load_elem(vb + i4 * ldv), acc, 0, 0, 0);
+                for (int q = 0; q < KB; ++q)
+                    ax[q] = (uint32_t)q < nkb ? *(const f4 *)(ga + 16 * q) : f4{0.f, 0.f, 0.f, 0.f};
             }
-            const uint32_t col = dt * 16 + row16;
-            if (a.dbg_max) {
-                if (dbg_on && col < a.d)
-                    for (int jj = 0; jj < 4; ++jj)
-                        a.dbg_du[(tile * OMAX + ot * 16 + 4 * grp + jj) * a.d + col] = acc[jj];
-            } else {
-                apply_tile<T, WB>(a, syn1, Us, ldv, td.rout_tok, n_out, ot * 16, col, lane, acc,
-                                  (tilesV + t) * 64 + lane, g0, td.sid);
+            for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                const uint32_t col = dt * 16 + l16;
+                const T *vb = Vs + g4 * ldv + col;
+                float bx[4 * KB];
+#pragma unroll
+                for (int q = 0; q < KB; ++q)
+                    if ((uint32_t)q < nkb)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) bx[4 * q + i] = load_elem(vb + (16 * q + i) * ldv);
+                f4 acc = {0.f, 0.f, 0.f, 0.f};
+                if (acc_old)
+                    for (int j = 0; j < 4; ++j) acc[j] = load_elem(Us + (o0 + j) * ldv + col);
+#pragma unroll
+                for (int q = 0; q < KB; ++q)
+                    if ((uint32_t)q < nkb)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[q][i], bx[4 * q + i], acc, 0, 0, 0);
+                if (a.dbg_max) {
+                    if (dbg_on && col < d)
+                        for (int j = 0; j < 4; ++j) a.dbg_du[(tile * OMAX + o0 + j) * d + col] = acc[j];
+                } else {
+                    store_tile<T, WB>(rows, Us + o0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                }
             }
         }
         ++my_tiles;

@@ -124,6 +124,43 @@ def test_sgns_groups_vs_fp64(er):
     print(f"worst relative error vs FP64: {worst:.3e}")
 
 
+def test_sgns_groups_bf16_vs_fp64():
+    """bf16 tables: X on bf16 MFMA from the exact bf16 rows; the gradient
+    products split G into bf16 hi + lo (relative residue below 2^-16), so the
+    deltas match FP64 over the same bf16 tables to 3e-5, the logits to 1e-5."""
+    src, dst, w = small_graph()
+    f = make_ctx(src, dst, w, 10_000, precision=felog.BF16)
+    try:
+        f.walk_round(1)
+        off, tok, _ = f.corpus()
+        n = 300
+        off_s, tok_s = off[: n + 1], tok[: int(off[n])]
+        s0, s1 = _random_tables(10_000, 128, 7)
+        f.set_embeddings(s0, s1)
+        b0, b1 = f.embeddings()  # the bf16 values the kernel reads
+        assert not np.array_equal(b0, s0)
+        got = f.debug_sgns(off_s, tok_s, 1, 0.025, 4096)
+        ref = orc.sgns_groups(_tables_for_oracle(f), off_s, tok_s, b0, b1, d=128, window=10, negative=5, batch=0,
+                              alpha=0.025, rnd=1, seed=SEED, mode=0)
+        assert len(got["groups"]) == ref["n_groups"] > 0
+        key = {(int(g[0]), int(g[1])): i for i, g in enumerate(ref["groups"])}
+        worst = {"logits": 0.0, "dv": 0.0, "du": 0.0}
+        tol = {"logits": 1e-5, "dv": 3e-5, "du": 3e-5}
+        for i, g in enumerate(got["groups"]):
+            j = key[(int(g[0]), int(g[1]))]
+            assert np.array_equal(g, ref["groups"][j])
+            n_in, n_out = int(g[3]), int(g[4])
+            for name, shape in (("logits", (n_in, n_out)), ("dv", (n_in, 128)), ("du", (n_out, 128))):
+                x = got[name][i][: shape[0], : shape[1]].astype(np.float64)
+                y = ref[name][j][: shape[0], : shape[1]]
+                err = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)
+                worst[name] = max(worst[name], err)
+                assert err < tol[name], (name, i, err)
+        print(f"bf16 worst relative error vs FP64: {worst}")
+    finally:
+        f.close()
+
+
 def test_eval_vs_oracle(er):
     f, _ = er
     rng = np.random.default_rng(3)
