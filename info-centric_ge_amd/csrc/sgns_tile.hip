@@ -54,11 +54,16 @@ __host__ __device__ constexpr uint32_t gt_pitch(uint32_t esize, uint32_t rmax) {
     return esize == 4 ? rmax + 4 : (rmax + 31) / 32 * 32 + 4;
 }
 
+// Groups of one tile with negatives (k >= 1): each has a centre and k
+// output slots, so at most OMAX / 2.
+constexpr uint32_t WROWS = OMAX / 2;
+
 // LDS: descriptor | RM input rows then OMAX output rows (pitch ldv) | the
-// OMAX x gt_pitch count / G^T tile.
+// OMAX-row count / G^T tile, then WROWS rows of per-group context counts
+// (same pitch) | per-output-row masks of negative slots.
 struct Lds {
     uint32_t ldv;  // row pitch of staged rows, in elements
-    size_t o_desc, o_rows, o_g, total;
+    size_t o_desc, o_rows, o_g, o_mask, total;
 };
 
 __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax) {
@@ -67,7 +72,8 @@ __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t 
     size_t o = 0;
     L.o_desc = o; o += kDescBytes;
     L.o_rows = o; o += (size_t)(rmax + OMAX) * L.ldv * esize;
-    L.o_g = o;    o += (size_t)OMAX * gt_pitch(esize, rmax) * 4;
+    L.o_g = o;    o += (size_t)(OMAX + WROWS) * gt_pitch(esize, rmax) * 4;
+    L.o_mask = o; o += (size_t)OMAX * 4;
     L.total = o;
     return L;
 }
@@ -102,19 +108,20 @@ __device__ __forceinline__ float pair_grad(uint32_t cc, float x, float alpha) {
     return alpha * (P - PQ * sigmoid_clamped(x));
 }
 
-// value of the adjacent lane (l ^ 1), by DPP quad permutation [1,0,3,2]
-__device__ __forceinline__ float swap_pair(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+// 32 rounding bits: Marsaglia's xorshift32 (13, 17, 5), seeded by one
+// Philox draw per lane and tile (a zero seed is replaced)
+__device__ __forceinline__ uint32_t round_bits(uint32_t &s) {
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
 }
 
-// 32 rounding bits: murmur3's finaliser over a Weyl sequence seeded by one
-// Philox draw per lane and tile
-__device__ __forceinline__ uint32_t round_bits(uint32_t base, uint32_t n) {
-    uint32_t h = base + n * 0x9E3779B9u;
-    h ^= h >> 16; h *= 0x85EBCA6Bu;
-    h ^= h >> 13; h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
+// upper 16 bits of x + r16 (r16 < 2^16): x stochastically rounded to bf16;
+// infinities and NaN are truncated unchanged
+__device__ __forceinline__ uint32_t sr_bits(float x, uint32_t r16) {
+    const uint32_t b = __float_as_uint(x);
+    return (__builtin_isfinite(x) ? b + r16 : b) >> 16;
 }
 
 // bf16 pair += (d0, d1) with stochastic rounding, by compare-and-swap on the
@@ -132,73 +139,57 @@ __device__ __forceinline__ void bf16_pair_add_sr(uint16_t *p, float d0, float d1
     }
 }
 
-// bf16 pair <- staged pair + (d0, d1), stochastically rounded, one store.
-__device__ __forceinline__ void bf16_pair_store_sr(uint16_t *p, const uint16_t *staged, float d0, float d1,
-                                                   uint32_t rnd) {
-    const float lo = bf16_to_f32(staged[0]) + d0;
-    const float hi = bf16_to_f32(staged[1]) + d1;
-    *reinterpret_cast<uint32_t *>(p) =
-        (uint32_t)f32_to_bf16_sr(lo, rnd) | ((uint32_t)f32_to_bf16_sr(hi, rnd >> 16) << 16);
-}
-
-// Table rows one lane writes for a 16-row block (MFMA output layout: lane l
-// holds rows r0..r0+3, r0 = 16*rt + 4*(l/16), of one column): all four in
-// f32; in bf16 adjacent columns pair up across lanes l, l^1 into one 32-bit
-// store, even lanes writing rows 0-1 of the four and odd lanes rows 2-3.
+// dVin and dUout are formed transposed (operands swapped in the MFMA), so
+// lane l holds row 16*blk + l%16 of the output and its four consecutive
+// columns col0 = 16*dt + 4*(l/16) .. col0+3: one table row per lane and one
+// contiguous store per 16x16 tile.
 template <typename T>
-struct LaneRows {
-    static constexpr int N = sizeof(T) == 4 ? 4 : 2;
-    T *p[N];          // table row
-    uint32_t rel[N];  // row within the lane's four
-    bool ok[N];       // row exists in the tile
+struct LaneRow {
+    T *p;     // table row
+    bool ok;  // row exists in the tile
 };
 
 template <typename T>
-__device__ __forceinline__ LaneRows<T> lane_rows(T *table, const uint32_t *ids, uint32_t r0, uint32_t n,
-                                                 uint32_t dp, uint32_t lane) {
-    LaneRows<T> R;
-#pragma unroll
-    for (int h = 0; h < LaneRows<T>::N; ++h) {
-        const uint32_t j = sizeof(T) == 4 ? (uint32_t)h : ((lane & 1) ? 2u + h : (uint32_t)h);
-        R.rel[h] = j;
-        R.ok[h] = r0 + j < n;
-        R.p[h] = table + (uint64_t)ids[R.ok[h] ? r0 + j : 0] * dp;
-    }
+__device__ __forceinline__ LaneRow<T> lane_row(T *table, const uint32_t *ids, uint32_t r, uint32_t n,
+                                               uint32_t dp) {
+    LaneRow<T> R;
+    R.ok = r < n;
+    R.p = table + (uint64_t)ids[R.ok ? r : 0] * dp;
     return R;
 }
 
-// Write one 16x16 output tile held in acc to the lane's rows.  f32
-// write-back: acc already holds the staged value plus the delta; f32 atomic:
-// acc is the delta.  bf16: staged pair + delta pair, stochastically rounded
-// (write-back), or added by compare-and-swap (atomic).
+// Write the lane's four values (columns col0..col0+3 of its row; padding
+// columns past d hold zero deltas on zero rows, so whole vectors are
+// written).  f32 write-back: acc already holds the staged value plus the
+// delta; f32 atomic: acc is the delta.  bf16 write-back: staged values plus
+// deltas, stochastically rounded, one 8-byte store; bf16 atomic: two
+// compare-and-swap pairs.
 template <typename T, bool WB>
-__device__ __forceinline__ void store_tile(const LaneRows<T> &R, const T *staged, uint32_t ldv, uint32_t col,
-                                           uint32_t d, uint32_t lane, const f4 &acc, uint32_t rbase,
-                                           uint32_t &rn) {
+__device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, uint32_t col0, const f4 &acc,
+                                          uint32_t &rs) {
+    if (!R.ok) return;
     if constexpr (sizeof(T) == 4) {
-        if (col >= d) return;
+        float *dst = (float *)R.p + col0;
+        if constexpr (WB) {
+            *(f4 *)dst = acc;
+        } else {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (!R.ok[j]) continue;
-            if constexpr (WB) R.p[j][col] = acc[j];
-            else unsafeAtomicAdd(&R.p[j][col], acc[j]);
+            for (int j = 0; j < 4; ++j) unsafeAtomicAdd(dst + j, acc[j]);
         }
     } else {
-        float other[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) other[j] = swap_pair(acc[j]);
-        const bool odd = lane & 1;
-        const uint32_t c2 = col & ~1u;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const uint32_t bits = round_bits(rbase, rn++);
-            const float lo_v = odd ? other[2 + h] : acc[h];
-            const float hi_v = odd ? acc[2 + h] : other[h];
-            if (!R.ok[h] || c2 >= d) continue;
-            if constexpr (WB)
-                bf16_pair_store_sr(R.p[h] + c2, staged + R.rel[h] * ldv + c2, lo_v, hi_v, bits);
-            else
-                bf16_pair_add_sr(R.p[h] + c2, lo_v, hi_v, bits);
+        uint16_t *dst = (uint16_t *)R.p + col0;
+        const uint32_t b0 = round_bits(rs), b1 = round_bits(rs);
+        if constexpr (WB) {
+            const uint2 old = *(const uint2 *)(staged + col0);
+            uint2 nw;
+            nw.x = sr_bits(__uint_as_float(old.x << 16) + acc[0], b0 & 0xFFFFu) |
+                   (sr_bits(__uint_as_float(old.x & 0xFFFF0000u) + acc[1], b0 >> 16) << 16);
+            nw.y = sr_bits(__uint_as_float(old.y << 16) + acc[2], b1 & 0xFFFFu) |
+                   (sr_bits(__uint_as_float(old.y & 0xFFFF0000u) + acc[3], b1 >> 16) << 16);
+            *(uint2 *)dst = nw;
+        } else {
+            bf16_pair_add_sr(dst, acc[0], acc[1], b0);
+            bf16_pair_add_sr(dst + 2, acc[2], acc[3], b1);
         }
     }
 }
@@ -217,6 +208,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     T *Us = Vs + RM * ldv;
     float *Gt = (float *)(lds + L.o_g);
     uint32_t *cnt = (uint32_t *)Gt;  // counts, overwritten in place by G^T
+    uint32_t *wcnt = cnt + OMAX * GT;  // per-group context counts W[g][r]
+    uint32_t *negmask = (uint32_t *)(lds + L.o_mask);
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t l16 = lane & 15, g4 = 4 * (lane >> 4);
@@ -233,8 +226,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     const uint32_t st_dr = kBlock / vpr, st_dc = kBlock % vpr;
     // pair counts: eight threads per centre, strided over its window
     const uint32_t ci = tid >> 3, cs = tid & 7, span = 2 * w + 1;
+    // negative slot q of a tile belongs to its group q / k (q < 32: the
+    // rounded float quotient is exact)
+    const float inv_k = k ? 1.f / (float)k : 0.f;
     // f32 write-back tiles start from the staged rows (acc = old + delta)
     const bool acc_old = WB && sizeof(T) == 4 && a.dbg_max == 0;
+    // stochastic-rounding stream of this thread for the launch (bf16 tables)
+    uint32_t rs = 0;
+    if constexpr (sizeof(T) == 2) {
+        rs = draw(a.seed, blockIdx.x * kBlock + tid, DOM_ROUND_BF16, 0u, a.round).x;
+        if (rs == 0) rs = 0x9E3779B9u;
+    }
 
     // bf16 products contract over whole 32-row blocks, reading staged rows
     // past a tile's own (rows of earlier tiles, finite); clear them once
@@ -244,23 +246,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     uint64_t n_tiles = a.n_sent + a.counters[C_TILES];
     if (n_tiles > a.tiles_cap) n_tiles = a.tiles_cap;
     const uint4 *descs = (const uint4 *)a.tiles;
-    uint64_t tile = blockIdx.x;
+    const uint32_t *desc_words = (const uint32_t *)a.tiles;
+
+    // Tiles of this workgroup: blockIdx.x + i * gridDim.x, skipping slots
+    // without a tile (nc = 0).  Every wave reads the nc words of the next 64
+    // candidates and ballots, so all waves walk the same live tiles with no
+    // barrier spent on empty slots.
+    uint64_t win = blockIdx.x, live = 0;
+    auto scan = [&]() {
+        const uint64_t t = win + (uint64_t)lane * gridDim.x;
+        live = __ballot(t < n_tiles && desc_words[t * (kDescBytes / 4) + 2] != 0u);
+    };
+    auto next_live = [&]() -> uint64_t {
+        while (live == 0) {
+            win += 64ull * gridDim.x;
+            if (win >= n_tiles) return n_tiles;
+            scan();
+        }
+        const uint64_t t = win + (uint64_t)__builtin_ctzll(live) * gridDim.x;
+        live &= live - 1;
+        return t;
+    };
+    if (win < n_tiles) scan();
+    uint64_t tile = win < n_tiles ? next_live() : n_tiles;
     uint4 pre = make_uint4(0, 0, 0, 0);
     if (tid < (uint32_t)kDescVec && tile < n_tiles) pre = descs[tile * kDescVec + tid];
 
-    for (; tile < n_tiles; tile += gridDim.x) {
+    for (uint64_t nxt; tile < n_tiles; tile = nxt) {
         if (tid < (uint32_t)kDescVec) ((uint4 *)&td)[tid] = pre;
-        for (uint32_t i = tid; i < OMAX * GT / 4; i += kBlock) ((uint4 *)cnt)[i] = make_uint4(0, 0, 0, 0);
+        for (uint32_t i = tid; i < ((OMAX + WROWS) * GT + OMAX) / 4; i += kBlock)
+            ((uint4 *)cnt)[i] = make_uint4(0, 0, 0, 0);  // counts, W and the masks
         __syncthreads();
+        nxt = next_live();
 
         const uint32_t c0 = td.c0, nc = td.nc, g0 = td.g0, lo = td.lo, npos = td.npos;
         const uint32_t n_in = td.n_in, n_out = td.n_out;
-        const uint64_t nxt = tile + gridDim.x;
-        if (nc == 0) {  // sentence without a tile
-            if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
-            __syncthreads();
-            continue;
-        }
         const uint32_t R16 = (n_in + 15) & ~15u, O16 = (n_out + 15) & ~15u;
         const uint32_t nrows = R16 + O16;
 
@@ -296,28 +316,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
         if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
 
         // ---- 2. pair counts: P in the low half-word, Q in the high -------------
+        // Q[o][r] = sum over the negative slots q holding row o of W[q/k][r]
+        // (pairs of q's group with context row r), less the pairs whose centre
+        // row is o itself (word2vec skips a negative equal to the centre).
+        // Per pair: P += 1 and the skipped negatives subtracted at [ro][r]
+        // (one atomic), W += 1 (one atomic); Q is completed from W and the
+        // slot masks when G is formed.
         if (ci < nc) {
             const uint32_t b = td.bshrink[ci];
             const uint32_t ro = td.cen_rout[ci];
-            const uint32_t nb = ((c0 + ci) / B - g0) * k;  // negatives of this centre's group
-            uint32_t neg[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) neg[j] = (uint32_t)j < k ? td.neg_rout[nb + j] : 0u;
+            const uint32_t grp = (c0 + ci) / B - g0;  // this centre's group within the tile
+            uint32_t coll = 0;
+            for (uint32_t j = 0; j < k; ++j) coll += td.neg_rout[grp * k + j] == ro;
+            const uint32_t pinc = 1u - (coll << 16);
+            uint32_t *wrow = wcnt + grp * GT;
             for (uint32_t aa = cs; aa < span; aa += 8) {
                 if (aa == w || aa < b || aa >= span - b) continue;
                 const int p = (int)(c0 + ci) - (int)w + (int)aa - (int)lo;  // input position in the tile
                 if (p < 0 || p >= (int)npos) continue;
                 const uint32_t rin = td.pos_rin[p];
-                atomicAdd(&cnt[ro * GT + rin], 1u);
-#pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((uint32_t)j < k && neg[j] != ro) atomicAdd(&cnt[neg[j] * GT + rin], 1u << 16);
-                for (uint32_t j = 8; j < k; ++j) {
-                    const uint32_t no = td.neg_rout[nb + j];
-                    if (no != ro) atomicAdd(&cnt[no * GT + rin], 1u << 16);
-                }
+                atomicAdd(&cnt[ro * GT + rin], pinc);
+                if (k) atomicAdd(&wrow[rin], 1u);
                 ++my_pairs;
             }
+        }
+        {
+            const uint32_t nslots = ((c0 + nc - 1) / B - g0 + 1) * k;  // negative slots of the tile
+            if (tid < nslots) atomicOr(&negmask[td.neg_rout[tid]], 1u << tid);
         }
 
         // ---- 3. gathered vectors to LDS; any further vectors directly ---------
@@ -386,7 +411,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
                 }
                 const uint32_t o = ot * 16 + l16, r0 = rt * 16 + g4;
                 uint32_t *cp = cnt + o * GT + r0;
-                const uint4 cc = *(const uint4 *)cp;
+                uint4 cc = *(const uint4 *)cp;
+                for (uint32_t m = negmask[o]; m; m &= m - 1) {  // negative slots holding row o
+                    const uint32_t g = (uint32_t)(((float)__builtin_ctz(m) + 0.5f) * inv_k);
+                    const uint4 wv = *(const uint4 *)(wcnt + g * GT + r0);
+                    cc.x += wv.x << 16;
+                    cc.y += wv.y << 16;
+                    cc.z += wv.z << 16;
+                    cc.w += wv.w << 16;
+                }
                 f4 gv;
                 gv[0] = pair_grad(cc.x, acc[0], a.alpha);
                 gv[1] = pair_grad(cc.y, acc[1], a.alpha);
@@ -411,36 +444,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
         }
         __syncthreads();
 
-        // rounding bits of this lane for the tile (bf16 tables)
-        uint32_t rbase = 0, rn = 0;
-        if constexpr (sizeof(T) == 2) rbase = draw(a.seed, tid, DOM_ROUND_BF16 | g0, td.sid, a.round).x;
-
         if constexpr (sizeof(T) == 2) {
             // ---- 5/6 on bf16 MFMA (16x16x32): G = hi + lo, two products each ----
+            // Operands are swapped (first = table rows, second = G) so the
+            // products come out transposed: lane = one row, four columns.
             lds_u8 *l3 = (lds_u8 *)lds;
             const uint32_t grp = lane >> 4, jq = l16 >> 2, qq = l16 & 3;
             const uint32_t gb = (uint32_t)L.o_g, rb = (uint32_t)L.o_rows, rowb = ldv * 2;
             // dVin = G * Uout, K = all OMAX output rows (zero G past n_out)
             for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
-                const uint32_t r0 = rt * 16 + g4;
-                const LaneRows<T> rows = lane_rows(syn0, td.rin_tok, r0, n_in, dp, lane);
-                // A = G[r][o], o = 8*grp + s, read transposed from G^T rows
+                const uint32_t r = rt * 16 + l16;
+                const LaneRow<T> row = lane_row(syn0, td.rin_tok, r, n_in, dp);
+                // G[r'][o], r' = 16*rt + lane%16, o = 8*grp + s: transposed from G^T
                 const uint32_t a0 = gb + (8 * grp + jq) * GT * 4 + (rt * 4 + qq) * 16;
-                const bf8 ahi = join8(tr_load(l3, a0), tr_load(l3, a0 + 16 * GT));
-                const bf8 alo = join8(tr_load(l3, a0 + 8), tr_load(l3, a0 + 16 * GT + 8));
+                const bf8 ghi = join8(tr_load(l3, a0), tr_load(l3, a0 + 16 * GT));
+                const bf8 glo = join8(tr_load(l3, a0 + 8), tr_load(l3, a0 + 16 * GT + 8));
                 for (uint32_t dt = wave; dt < dtiles; dt += 4) {
-                    const uint32_t col = dt * 16 + l16;
-                    // B = Uout[o][col], read transposed from the staged rows
+                    const uint32_t col0 = dt * 16 + g4;
+                    // Uout[o][16*dt + lane%16], transposed from the staged rows
                     const uint32_t b0 = rb + (RM + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
-                    const bf8 bv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
+                    const bf8 uv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
                     f4 acc = {0.f, 0.f, 0.f, 0.f};
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bv, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bv, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, ghi, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, glo, acc, 0, 0, 0);
                     if (a.dbg_max) {
-                        if (dbg_on && col < d)
-                            for (int j = 0; j < 4; ++j) a.dbg_dv[(tile * RMAX + r0 + j) * d + col] = acc[j];
+                        if (dbg_on)
+                            for (int j = 0; j < 4; ++j)
+                                if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
                     } else {
-                        store_tile<T, WB>(rows, Vs + r0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                        store_row<T, WB>(row, Vs + r * ldv, col0, acc, rs);
                     }
                 }
             }
@@ -448,35 +480,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
             // input rows past RM are output rows, finite)
             const uint32_t nk32 = (R16 + 31) >> 5;
             for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
-                const uint32_t o0 = ot * 16 + g4;
-                const LaneRows<T> rows = lane_rows(syn1, td.rout_tok, o0, n_out, dp, lane);
-                // A = G^T[o][r], r = 32*kb + 8*grp + s: two 4-row blocks
-                bf8 ahi[2], alo[2];
+                const uint32_t o = ot * 16 + l16;
+                const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
+                // G^T[o][r], r = 32*kb + 8*grp + s: two 4-row blocks
+                bf8 ghi[2], glo[2];
 #pragma unroll
                 for (int kb = 0; kb < 2; ++kb) {
-                    const uint4 *gp = (const uint4 *)(lds + gb + ((ot * 16 + l16) * GT + 32 * kb + 8 * grp) * 4);
+                    const uint4 *gp = (const uint4 *)(lds + gb + (o * GT + 32 * kb + 8 * grp) * 4);
                     const uint4 x = (uint32_t)kb < nk32 ? gp[0] : make_uint4(0, 0, 0, 0);
                     const uint4 y = (uint32_t)kb < nk32 ? gp[1] : make_uint4(0, 0, 0, 0);
-                    ahi[kb] = __builtin_bit_cast(bf8, u4v{x.x, x.y, y.x, y.y});
-                    alo[kb] = __builtin_bit_cast(bf8, u4v{x.z, x.w, y.z, y.w});
+                    ghi[kb] = __builtin_bit_cast(bf8, u4v{x.x, x.y, y.x, y.y});
+                    glo[kb] = __builtin_bit_cast(bf8, u4v{x.z, x.w, y.z, y.w});
                 }
                 for (uint32_t dt = wave; dt < dtiles; dt += 4) {
-                    const uint32_t col = dt * 16 + l16;
+                    const uint32_t col0 = dt * 16 + g4;
                     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int kb = 0; kb < 2; ++kb) {
                         if ((uint32_t)kb >= nk32) break;
-                        // B = Vin[r][col], read transposed from the staged rows
+                        // Vin[r][16*dt + lane%16], transposed from the staged rows
                         const uint32_t b0 = rb + (32 * kb + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
-                        const bf8 bv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[kb], bv, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[kb], bv, acc, 0, 0, 0);
+                        const bf8 vv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, ghi[kb], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, glo[kb], acc, 0, 0, 0);
                     }
                     if (a.dbg_max) {
-                        if (dbg_on && col < d)
-                            for (int j = 0; j < 4; ++j) a.dbg_du[(tile * OMAX + o0 + j) * d + col] = acc[j];
+                        if (dbg_on)
+                            for (int j = 0; j < 4; ++j)
+                                if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
                     } else {
-                        store_tile<T, WB>(rows, Us + o0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                        store_row<T, WB>(row, Us + o * ldv, col0, acc, rs);
                     }
                 }
             }
@@ -485,85 +518,84 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
             continue;
         }
 
-        // ---- 5. dVin = G * Uout  (R16 x dp, K = O16) ----------------------------
+        // ---- 5/6 on f32 MFMA (16x16x4), operands swapped as above -------------
+        // dVin = G * Uout  (R16 x dp, K = O16)
         for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
-            const uint32_t r0 = rt * 16 + g4;
-            const LaneRows<T> rows = lane_rows(syn0, td.rin_tok, r0, n_in, dp, lane);
-            // A operand G[r][k], k = 16*kb' + 4*(lane/16) + i: one column of
-            // G^T per k, the same for every column block of this row block
+            const uint32_t r = rt * 16 + l16;
+            const LaneRow<T> row = lane_row(syn0, td.rin_tok, r, n_in, dp);
+            // G[r'][k], k = 16*kb' + 4*(lane/16) + i: one column of G^T per k,
+            // the same for every column block of this row block
             const bool o2 = O16 > 16;
-            float ax[8];
+            float gx[8];
             {
                 const float *ga = Gt + g4 * GT + rt * 16 + l16;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) ax[i] = ga[i * GT];
+                for (int i = 0; i < 4; ++i) gx[i] = ga[i * GT];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) ax[4 + i] = o2 ? ga[(16 + i) * GT] : 0.f;
+                for (int i = 0; i < 4; ++i) gx[4 + i] = o2 ? ga[(16 + i) * GT] : 0.f;
             }
             for (uint32_t dt = wave; dt < dtiles; dt += 4) {
-                const uint32_t col = dt * 16 + l16;
-                const T *ub = Us + g4 * ldv + col;
-                float bx[8];
+                const uint32_t col0 = dt * 16 + g4;
+                const T *ub = Us + g4 * ldv + dt * 16 + l16;
+                float ux[8];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) bx[i] = load_elem(ub + i * ldv);
+                for (int i = 0; i < 4; ++i) ux[i] = load_elem(ub + i * ldv);
                 if (o2)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) bx[4 + i] = load_elem(ub + (16 + i) * ldv);
-                f4 acc = {0.f, 0.f, 0.f, 0.f};
-                if (acc_old)
-                    for (int j = 0; j < 4; ++j) acc[j] = load_elem(Vs + (r0 + j) * ldv + col);
+                    for (int i = 0; i < 4; ++i) ux[4 + i] = load_elem(ub + (16 + i) * ldv);
+                f4 acc = acc_old ? *(const f4 *)(Vs + r * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[i], bx[i], acc, 0, 0, 0);
+                for (int i = 0; i < 4; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ux[i], gx[i], acc, 0, 0, 0);
                 if (o2)
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[4 + i], bx[4 + i], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ux[4 + i], gx[4 + i], acc, 0, 0, 0);
                 if (a.dbg_max) {
-                    if (dbg_on && col < d)
-                        for (int j = 0; j < 4; ++j) a.dbg_dv[(tile * RMAX + r0 + j) * d + col] = acc[j];
+                    if (dbg_on)
+                        for (int j = 0; j < 4; ++j)
+                            if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
                 } else {
-                    store_tile<T, WB>(rows, Vs + r0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                    store_row<T, WB>(row, Vs + r * ldv, col0, acc, rs);
                 }
             }
         }
-        // ---- 6. dUout = G^T * Vin  (O16 x dp, K = R16) --------------------------
+        // dUout = G^T * Vin  (O16 x dp, K = R16)
         for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
-            const uint32_t o0 = ot * 16 + g4;
-            const LaneRows<T> rows = lane_rows(syn1, td.rout_tok, o0, n_out, dp, lane);
-            // A operand G^T[o][k]: four consecutive words per 16-deep block,
-            // the same for every column block of this row block
+            const uint32_t o = ot * 16 + l16;
+            const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
+            // G^T[o][k]: four consecutive words per 16-deep block, the same
+            // for every column block of this row block
             constexpr int KB = RM / 16;
             const uint32_t nkb = R16 >> 4;
-            f4 ax[KB];
+            f4 gx[KB];
             {
-                const float *ga = Gt + (ot * 16 + l16) * GT + g4;
+                const float *ga = Gt + o * GT + g4;
 #pragma unroll
                 for (int q = 0; q < KB; ++q)
-                    ax[q] = (uint32_t)q < nkb ? *(const f4 *)(ga + 16 * q) : f4{0.f, 0.f, 0.f, 0.f};
+                    gx[q] = (uint32_t)q < nkb ? *(const f4 *)(ga + 16 * q) : f4{0.f, 0.f, 0.f, 0.f};
             }
             for (uint32_t dt = wave; dt < dtiles; dt += 4) {
-                const uint32_t col = dt * 16 + l16;
-                const T *vb = Vs + g4 * ldv + col;
-                float bx[4 * KB];
+                const uint32_t col0 = dt * 16 + g4;
+                const T *vb = Vs + g4 * ldv + dt * 16 + l16;
+                float vx[4 * KB];
 #pragma unroll
                 for (int q = 0; q < KB; ++q)
                     if ((uint32_t)q < nkb)
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) bx[4 * q + i] = load_elem(vb + (16 * q + i) * ldv);
-                f4 acc = {0.f, 0.f, 0.f, 0.f};
-                if (acc_old)
-                    for (int j = 0; j < 4; ++j) acc[j] = load_elem(Us + (o0 + j) * ldv + col);
+                        for (int i = 0; i < 4; ++i) vx[4 * q + i] = load_elem(vb + (16 * q + i) * ldv);
+                f4 acc = acc_old ? *(const f4 *)(Us + o * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int q = 0; q < KB; ++q)
                     if ((uint32_t)q < nkb)
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
-                            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[q][i], bx[4 * q + i], acc, 0, 0, 0);
+                            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(vx[4 * q + i], gx[q][i], acc, 0, 0, 0);
                 if (a.dbg_max) {
-                    if (dbg_on && col < d)
-                        for (int j = 0; j < 4; ++j) a.dbg_du[(tile * OMAX + o0 + j) * d + col] = acc[j];
+                    if (dbg_on)
+                        for (int j = 0; j < 4; ++j)
+                            if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
                 } else {
-                    store_tile<T, WB>(rows, Us + o0 * ldv, ldv, col, d, lane, acc, rbase, rn);
+                    store_row<T, WB>(row, Us + o * ldv, col0, acc, rs);
                 }
             }
         }
