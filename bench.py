@@ -125,7 +125,7 @@ def main():
           file=sys.stderr, flush=True)
     prec = felog.BF16 if args.precision == "bf16" else felog.FP32
     esize = 2 if prec == felog.BF16 else 4
-    f = felog.Felog(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, alpha=0.025, min_L=20,
+    f = felog.Felog(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20,
                     min_R=2, precision=prec, seed=args.seed)
     f.set_edges(src, dst, w, n, make_undirected=True)
     f.build()

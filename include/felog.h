@@ -71,7 +71,8 @@ typedef struct felog_options {
     uint32_t batch_size;       /* -batch-size: centres sharing one negative
                                   set (0 -> 2*window+1, include/dsgl.hpp:832) */
     float sample;              /* -sample (1e-3)                             */
-    float alpha;               /* -alpha  (0.025)                            */
+    float alpha;               /* -alpha: base of the per-round LR (0.0025;
+                                  word2vec's default is 0.025)               */
     uint32_t min_count;        /* -min-count (5)                             */
     int32_t binary;            /* -binary (0)                                */
     /* build switches */

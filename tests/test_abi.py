@@ -33,10 +33,11 @@ def test_library_exports_every_declared_symbol():
 def test_default_options_are_the_reference_defaults():
     o = felog.default_options()
     # include/walk.hpp:416-417 (20, 5); src/word2vec.cu:162-168 (size 100,
-    # window 5, negative 5, sample 1e-3, alpha 0.025, min-count 5);
-    # include/option_helper.hpp:138 (--batch_size 16384)
+    # window 5, negative 5, sample 1e-3, min-count 5);
+    # include/option_helper.hpp:138 (--batch_size 16384).  alpha defaults to
+    # 0.0025, a tenth of word2vec's 0.025: tiles sum their pair gradients.
     assert (o.min_L, o.min_R, o.size, o.window, o.negative, o.min_count, o.eval_batch) == (20, 5, 100, 5, 5, 5, 16384)
-    assert abs(o.sample - 1e-3) < 1e-9 and abs(o.alpha - 0.025) < 1e-9
+    assert abs(o.sample - 1e-3) < 1e-9 and abs(o.alpha - 0.0025) < 1e-9
     assert o.max_walk_len == 1000 and o.batch_size == 0 and o.precision == felog.FP32
 
 
