@@ -205,6 +205,61 @@ felog_status felog_write_embeddings(felog_ctx *ctx, const char *path,
 
 felog_status felog_stats_get(felog_ctx *ctx, felog_stats *out);
 
+/* ---- learned walk budget (no counterpart in the reference) ------------- */
+/* An 8-64-64-2 ReLU MLP maps per-vertex features to (ln walk count, ln walk
+ * length), both standardised.  It is trained on the budgets the plain
+ * information-centric rule produced on other graphs; a vertex whose
+ * standardised features leave the training box [feat_lo, feat_hi] keeps the
+ * plain rule. */
+#define FELOG_BUDGET_FEATURES 8
+#define FELOG_BUDGET_PARAMS 4866  /* W1 64x8, b1, W2 64x64, b2, W3 2x64, b3 */
+typedef struct felog_budget_model {
+    float params[FELOG_BUDGET_PARAMS];
+    float feat_mean[FELOG_BUDGET_FEATURES], feat_std[FELOG_BUDGET_FEATURES];
+    float feat_lo[FELOG_BUDGET_FEATURES], feat_hi[FELOG_BUDGET_FEATURES];
+    float targ_mean[2], targ_std[2];  /* of ln(count), ln(length)        */
+} felog_budget_model;
+
+/* Raw features of every vertex (V x 8, after felog_build): ln(1+deg),
+ * ln(1+sum w), max w, mean w, ln(1+sum of neighbour degrees), ln(1+max
+ * neighbour degree), transition entropy / ln deg, in-vocabulary flag. */
+felog_status felog_budget_features(felog_ctx *ctx, float *features);
+/* Mean-squared-error gradient over n samples of standardised features X
+ * (n x 8) and targets T (n x 2) at params; *loss = the mean squared error. */
+felog_status felog_budget_grad(felog_ctx *ctx, const float *X, const float *T,
+                               uint64_t n, const float *params, float *grad,
+                               double *loss);
+/* Adam training from params (updated in place): epochs x ceil(n/batch)
+ * steps of batch samples drawn with replacement (batch a multiple of 64);
+ * loss_per_epoch (may be NULL) receives each epoch's minibatch mean loss. */
+felog_status felog_budget_train(felog_ctx *ctx, const float *X, const float *T,
+                                uint64_t n, uint32_t epochs, uint32_t batch,
+                                float lr, uint64_t seed, float *params,
+                                double *loss_per_epoch);
+/* Install a model: predicts every vertex's budget now; later walk rounds
+ * start budgeted vertices in rounds 1..count with walks of exactly length
+ * vertices (no entropy state) and leave the rest to the plain rule.
+ * model NULL removes the budget. */
+felog_status felog_budget_set(felog_ctx *ctx, const felog_budget_model *model);
+/* Predicted budgets (0 = plain rule); either pointer may be NULL.  *rounds
+ * receives the largest count. */
+felog_status felog_budget_copy_out(felog_ctx *ctx, uint16_t *count,
+                                   uint16_t *length, uint32_t *rounds);
+/* Rounds of the budgeted run: walks and training until every budgeted
+ * vertex has used its count and (when plain-rule vertices remain) the plain
+ * stop rule fires; evaluation runs only for plain-rule vertices. */
+felog_status felog_run_budget(felog_ctx *ctx);
+
+/* ---- visit diagnostics (include/walk.hpp:754-788) ---------------------- */
+/* Start (or restart) counting corpus visits; each later walk round adds its
+ * tokens to a per-vertex histogram. */
+felog_status felog_visits_reset(felog_ctx *ctx);
+/* From the histogram q (visit shares) and p (degree shares):
+ * out[0] = KL(p || q) = sum p (ln p - ln q) over visited vertices,
+ * out[1] = Shannon entropy -sum q ln q (nats), out[2] = vertices with p > 0
+ * and no visit, out[3] = visits counted. */
+felog_status felog_visit_stats(felog_ctx *ctx, double out[4]);
+
 /* ---- test hook --------------------------------------------------------- */
 /* Runs the SGNS kernels on a caller-supplied corpus with updates diverted:
  * instead of touching the tables they write, for every tile (a run of

@@ -63,7 +63,8 @@ void free_all(felog_ctx *c) {
                     c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni_thr,
                     c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
                     c->d_scores, c->d_active, c->d_walk_ok, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
-                    c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp};
+                    c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
+                    c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats};
     for (void *p : ptrs) dfree(p);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -283,6 +284,7 @@ felog_status felog_walk_round(felog_ctx *c, uint32_t round, felog_corpus *out) {
     const uint64_t maxlen = c->h_counters[C_MAXLEN];
     c->st.overflow_used = c->h_counters[C_OVF_NEXT];
     TRY(c, compact_corpus(c, (uint32_t)c->corpus.n_walks, &cms));
+    if (c->d_visits) TRY(c, add_visits(c, c->d_tok, c->corpus.n_tokens));
     c->corpus.round = round;
     c->corpus.n_steps = steps;
     c->corpus.max_len = (uint32_t)maxlen;
@@ -447,6 +449,192 @@ felog_status felog_run(felog_ctx *c) {
         }
         if (corpus.n_walks == 0) break;
     }
+    return FELOG_OK;
+}
+
+// ---- learned walk budget ------------------------------------------------
+
+felog_status felog_budget_features(felog_ctx *c, float *features) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!features) return fail(c, FELOG_ERR_ARG, "null features buffer");
+    const uint64_t bytes = (uint64_t)c->V * FELOG_BUDGET_FEATURES * 4;
+    if (!c->d_feat) TRY(c, hipMalloc(&c->d_feat, bytes));
+    TRY(c, budget_features(c, c->d_feat));
+    TRY(c, hipMemcpyAsync(features, c->d_feat, bytes, hipMemcpyDeviceToHost, c->stream));
+    TRY(c, hipStreamSynchronize(c->stream));
+    return FELOG_OK;
+}
+
+namespace {
+// Device copies of a training set and the optimiser state; freed on return.
+struct MlpBufs {
+    float *x = nullptr, *t = nullptr, *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    double *loss = nullptr;
+    ~MlpBufs() {
+        for (void *q : {(void *)x, (void *)t, (void *)p, (void *)g, (void *)m, (void *)v, (void *)loss}) dfree(q);
+    }
+};
+}  // namespace
+
+felog_status felog_budget_grad(felog_ctx *c, const float *X, const float *T, uint64_t n, const float *params,
+                               float *grad, double *loss) {
+    if (!c) return FELOG_ERR_ARG;
+    if (!X || !T || !params || !grad || n == 0) return fail(c, FELOG_ERR_ARG, "budget_grad: bad arguments");
+    TRY(c, hipSetDevice(c->device));
+    MlpBufs b;
+    const size_t P = FELOG_BUDGET_PARAMS * 4;
+    TRY(c, hipMalloc(&b.x, n * FELOG_BUDGET_FEATURES * 4));
+    TRY(c, hipMalloc(&b.t, n * 2 * 4));
+    TRY(c, hipMalloc(&b.p, P));
+    TRY(c, hipMalloc(&b.g, P));
+    TRY(c, hipMalloc(&b.loss, 8));
+    TRY(c, hipMemcpyAsync(b.x, X, n * FELOG_BUDGET_FEATURES * 4, hipMemcpyHostToDevice, c->stream));
+    TRY(c, hipMemcpyAsync(b.t, T, n * 2 * 4, hipMemcpyHostToDevice, c->stream));
+    TRY(c, hipMemcpyAsync(b.p, params, P, hipMemcpyHostToDevice, c->stream));
+    TRY(c, hipMemsetAsync(b.g, 0, P, c->stream));
+    TRY(c, hipMemsetAsync(b.loss, 0, 8, c->stream));
+    const uint64_t blocks = (n + 63) / 64;
+    if (blocks > 0x7FFFFFFFull) return fail(c, FELOG_ERR_ARG, "budget_grad: too many samples");
+    TRY(c, budget_grad(c, b.x, b.t, n, 0, 0, (uint32_t)blocks, b.p, b.g, b.loss, 1.f / (float)n));
+    double l = 0.0;
+    TRY(c, hipMemcpyAsync(grad, b.g, P, hipMemcpyDeviceToHost, c->stream));
+    TRY(c, hipMemcpyAsync(&l, b.loss, 8, hipMemcpyDeviceToHost, c->stream));
+    TRY(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l / (double)n;
+    return FELOG_OK;
+}
+
+felog_status felog_budget_train(felog_ctx *c, const float *X, const float *T, uint64_t n, uint32_t epochs,
+                                uint32_t batch, float lr, uint64_t seed, float *params, double *loss_per_epoch) {
+    if (!c) return FELOG_ERR_ARG;
+    if (!X || !T || !params || n == 0 || epochs == 0 || batch == 0 || batch % 64 || !(lr > 0.f))
+        return fail(c, FELOG_ERR_ARG, "budget_train: bad arguments (batch must be a positive multiple of 64)");
+    TRY(c, hipSetDevice(c->device));
+    MlpBufs b;
+    const size_t P = FELOG_BUDGET_PARAMS * 4;
+    TRY(c, hipMalloc(&b.x, n * FELOG_BUDGET_FEATURES * 4));
+    TRY(c, hipMalloc(&b.t, n * 2 * 4));
+    TRY(c, hipMalloc(&b.p, P));
+    TRY(c, hipMalloc(&b.g, P));
+    TRY(c, hipMalloc(&b.m, P));
+    TRY(c, hipMalloc(&b.v, P));
+    TRY(c, hipMalloc(&b.loss, 8));
+    TRY(c, hipMemcpyAsync(b.x, X, n * FELOG_BUDGET_FEATURES * 4, hipMemcpyHostToDevice, c->stream));
+    TRY(c, hipMemcpyAsync(b.t, T, n * 2 * 4, hipMemcpyHostToDevice, c->stream));
+    TRY(c, hipMemcpyAsync(b.p, params, P, hipMemcpyHostToDevice, c->stream));
+    TRY(c, hipMemsetAsync(b.g, 0, P, c->stream));
+    TRY(c, hipMemsetAsync(b.m, 0, P, c->stream));
+    TRY(c, hipMemsetAsync(b.v, 0, P, c->stream));
+    const uint64_t steps = (n + batch - 1) / batch;
+    const uint64_t key = seed ? seed : 1;
+    uint32_t t = 0;
+    for (uint32_t e = 0; e < epochs; ++e) {
+        TRY(c, hipMemsetAsync(b.loss, 0, 8, c->stream));
+        for (uint64_t k = 0; k < steps; ++k) {
+            ++t;
+            TRY(c, budget_grad(c, b.x, b.t, n, key, t, batch / 64, b.p, b.g, b.loss, 1.f / (float)batch));
+            TRY(c, budget_adam(c, b.p, b.g, b.m, b.v, lr, t));
+        }
+        if (loss_per_epoch) {
+            double l = 0.0;
+            TRY(c, hipMemcpyAsync(&l, b.loss, 8, hipMemcpyDeviceToHost, c->stream));
+            TRY(c, hipStreamSynchronize(c->stream));
+            loss_per_epoch[e] = l / ((double)steps * batch);
+        }
+    }
+    TRY(c, hipMemcpyAsync(params, b.p, P, hipMemcpyDeviceToHost, c->stream));
+    TRY(c, hipStreamSynchronize(c->stream));
+    return FELOG_OK;
+}
+
+felog_status felog_budget_set(felog_ctx *c, const felog_budget_model *model) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!model) {
+        c->budget_on = false;
+        c->budget_rounds = 0;
+        return FELOG_OK;
+    }
+    const uint64_t fbytes = (uint64_t)c->V * FELOG_BUDGET_FEATURES * 4;
+    if (!c->d_feat) TRY(c, hipMalloc(&c->d_feat, fbytes));
+    if (!c->d_model) TRY(c, hipMalloc(&c->d_model, sizeof(felog_budget_model)));
+    if (!c->d_budget_cnt) TRY(c, hipMalloc(&c->d_budget_cnt, (uint64_t)c->V * 2));
+    if (!c->d_budget_len) TRY(c, hipMalloc(&c->d_budget_len, (uint64_t)c->V * 2));
+    TRY(c, hipMemcpyAsync(c->d_model, model, sizeof(felog_budget_model), hipMemcpyHostToDevice, c->stream));
+    TRY(c, budget_features(c, c->d_feat));
+    TRY(c, budget_predict(c, c->d_feat, c->d_model));
+    std::vector<uint16_t> cnt(c->V);
+    TRY(c, hipMemcpyAsync(cnt.data(), c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost, c->stream));
+    TRY(c, hipStreamSynchronize(c->stream));
+    uint32_t mx = 0;
+    for (uint16_t x : cnt) mx = x > mx ? x : mx;
+    c->budget_rounds = mx;
+    c->budget_on = true;
+    return FELOG_OK;
+}
+
+felog_status felog_budget_copy_out(felog_ctx *c, uint16_t *count, uint16_t *length, uint32_t *rounds) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->budget_on) return fail(c, FELOG_ERR_STATE, "no budget installed");
+    if (count) TRY(c, hipMemcpy(count, c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
+    if (length) TRY(c, hipMemcpy(length, c->d_budget_len, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
+    if (rounds) *rounds = c->budget_rounds;
+    return FELOG_OK;
+}
+
+felog_status felog_run_budget(felog_ctx *c) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->budget_on) return fail(c, FELOG_ERR_STATE, "no budget installed");
+    // plain-rule vertices: those with count 0
+    std::vector<uint16_t> cnt(c->V);
+    TRY(c, hipMemcpy(cnt.data(), c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
+    bool plain_left = false;
+    for (uint16_t x : cnt) plain_left = plain_left || x == 0;
+    bool plain_stopped = !plain_left;
+    for (uint32_t r = c->rounds_done + 1;; ++r) {
+        if (c->opt.max_rounds && r > c->opt.max_rounds) break;
+        if (r > c->budget_rounds && plain_stopped) break;
+        felog_corpus corpus;
+        if ((s = felog_walk_round(c, r, &corpus))) return s;
+        if (corpus.n_walks == 0) break;
+        if ((s = felog_train_round(c, &corpus))) return s;
+        if (!plain_stopped && r >= c->opt.min_R) {
+            uint32_t nu = 0;
+            int stop = 0;
+            if ((s = felog_evaluate(c, &nu, &stop))) return s;
+            if (stop) plain_stopped = true;
+        }
+    }
+    return FELOG_OK;
+}
+
+felog_status felog_visits_reset(felog_ctx *c) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->d_visits) TRY(c, hipMalloc(&c->d_visits, (uint64_t)c->V * 4));
+    if (!c->d_vstats) TRY(c, hipMalloc(&c->d_vstats, 4 * sizeof(double)));
+    TRY(c, hipMemsetAsync(c->d_visits, 0, (uint64_t)c->V * 4, c->stream));
+    TRY(c, hipStreamSynchronize(c->stream));
+    c->visits_total = 0;
+    return FELOG_OK;
+}
+
+felog_status felog_visit_stats(felog_ctx *c, double out[4]) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!out) return fail(c, FELOG_ERR_ARG, "null output");
+    if (!c->d_visits) return fail(c, FELOG_ERR_STATE, "visits are not tracked: call felog_visits_reset first");
+    TRY(c, visit_stats(c, c->d_vstats));
+    double h[3] = {0, 0, 0};
+    TRY(c, hipMemcpyAsync(h, c->d_vstats, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TRY(c, hipStreamSynchronize(c->stream));
+    out[0] = h[0];
+    out[1] = h[1];
+    out[2] = h[2];
+    out[3] = (double)c->visits_total;
     return FELOG_OK;
 }
 

@@ -106,6 +106,17 @@ struct felog_ctx {
     void *d_tmp = nullptr;
     size_t tmp_bytes = 0;
 
+    // learned walk budget: per-vertex walk count and length (0: plain rule)
+    bool budget_on = false;
+    uint16_t *d_budget_cnt = nullptr, *d_budget_len = nullptr;  // V
+    felog_budget_model *d_model = nullptr;
+    float *d_feat = nullptr;             // V x FELOG_BUDGET_FEATURES
+    uint32_t budget_rounds = 0;          // largest predicted count
+    // corpus visit histogram (cumulative over walk rounds while tracked)
+    uint32_t *d_visits = nullptr;        // V
+    uint64_t visits_total = 0;
+    double *d_vstats = nullptr;          // 3 doubles
+
     felog_stats st{};
 };
 
@@ -149,6 +160,16 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
                     uint64_t n_tok, uint32_t round, float alpha, const SgnsDebug *dbg, float *ms);
 // eval_kernel.hip
 hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms);
+// budget_kernels.hip
+constexpr uint32_t kBudgetMaxCount = 255;  // rounds a budget may assign
+hipError_t budget_features(felog_ctx *c, float *d_out);
+hipError_t budget_grad(felog_ctx *c, const float *d_x, const float *d_t, uint64_t n, uint64_t sample_seed,
+                       uint32_t step, uint32_t blocks, const float *d_p, float *d_grad, double *d_loss, float inv_n);
+hipError_t budget_adam(felog_ctx *c, float *d_p, float *d_grad, float *d_m, float *d_v, float lr, uint32_t t);
+hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model);
+hipError_t budget_walk_ok(felog_ctx *c, uint32_t round);
+hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok);
+hipError_t visit_stats(felog_ctx *c, double *d_out);
 hipError_t init_tables(felog_ctx *c);
 
 uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative);

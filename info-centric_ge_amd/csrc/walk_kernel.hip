@@ -12,6 +12,8 @@
 //     FP64 on the lane: prior visits f of the entered vertex come from the
 //     walker's history (first 32 vertices in LDS, the rest from its path in
 //     global memory);
+//   - a walker whose start vertex carries a learned length budget runs
+//     exactly that many vertices and keeps no entropy state;
 //   - reference special cases kept: degree 0 ends a walk (:964-969), a
 //     degree-2 vertex whose first neighbour is itself ends it (:977-979), a
 //     self loop drawn at step 0 is redrawn (:981-983, at most 64 times).
@@ -41,6 +43,7 @@ struct WalkArgs {
     unsigned long long *counters;
     const double *__restrict__ T;
     const double *__restrict__ dT;
+    const uint16_t *__restrict__ fixed_len;  // per start vertex, 0: stop rule (null: none)
     uint32_t min_L, max_len, round;
     uint64_t seed;
 };
@@ -60,7 +63,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
 
     bool active = false;
     bool more = true;  // wave-uniform: the work cursor may still hold walkers
-    uint32_t w = 0, v = 0, s = 0, attempt = 0;
+    uint32_t w = 0, v = 0, s = 0, attempt = 0, lim = 0;
     uint64_t ob = kNoOverflow;
     WalkStats st;
     walk_stats_init(st);
@@ -81,6 +84,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                 if (idx < a.n_active) {
                     w = a.active[idx];
                     v = w % a.V;
+                    lim = a.fixed_len ? a.fixed_len[v] : 0u;
                     s = 0;
                     attempt = 0;
                     ob = kNoOverflow;
@@ -94,7 +98,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
         if (__ballot(active) == 0) break;
         if (active) {
         bool stop = false;
-        if (s + 1 >= a.max_len) {
+        if (s + 1 >= a.max_len || (lim && s + 1 >= lim)) {
             stop = true;
         } else {
             const VertexRec vr = a.vrec[v];
@@ -111,11 +115,13 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                     if (++attempt >= kMaxStartAttempts) stop = true;
                 } else {
                     attempt = 0;
-                    // prior visits of x on this walk
+                    // prior visits of x on this walk (stop-rule walks only)
                     uint32_t f = 0;
-                    const uint32_t nl = s + 1 < (uint32_t)kHistLds ? s + 1 : (uint32_t)kHistLds;
-                    for (uint32_t k = 0; k < nl; ++k) f += (H[k * 64] == x) ? 1u : 0u;
-                    for (uint32_t k = kHistLds; k <= s; ++k) f += (path_at(a, w, ob, k) == x) ? 1u : 0u;
+                    if (!lim) {
+                        const uint32_t nl = s + 1 < (uint32_t)kHistLds ? s + 1 : (uint32_t)kHistLds;
+                        for (uint32_t k = 0; k < nl; ++k) f += (H[k * 64] == x) ? 1u : 0u;
+                        for (uint32_t k = kHistLds; k <= s; ++k) f += (path_at(a, w, ob, k) == x) ? 1u : 0u;
+                    }
                     const uint32_t ns = s + 1;
                     bool stored = true;
                     if (ns < a.slot) {
@@ -134,7 +140,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                     } else {
                         if (ns < (uint32_t)kHistLds) H[ns * 64] = x;
                         s = ns;
-                        stop = walk_stats_step(st, a.T, a.dT, f, s + 1, s, a.min_L);
+                        stop = lim ? s + 1 >= lim : walk_stats_step(st, a.T, a.dT, f, s + 1, s, a.min_L);
                         v = x;
                     }
                 }
@@ -214,8 +220,12 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap) {
 hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active_unused, float *walk_ms) {
     (void)n_active_unused;
     const uint32_t W = c->walkers;
-    k_walk_ok<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_flags, W, c->V, c->d_walk_ok);
-    HIP_TRY(hipGetLastError());
+    if (c->budget_on) {
+        HIP_TRY(budget_walk_ok(c, round));
+    } else {
+        k_walk_ok<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_flags, W, c->V, c->d_walk_ok);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipMemsetAsync(c->d_counters, 0, kNumCounters * 8, c->stream));
     size_t tmp = 0;
     hipcub::CountingInputIterator<uint32_t> ids(0);
@@ -254,6 +264,7 @@ hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active_unused, flo
     a.counters = (unsigned long long *)c->d_counters;
     a.T = c->d_T;
     a.dT = c->d_dT;
+    a.fixed_len = c->budget_on ? c->d_budget_len : nullptr;
     a.min_L = c->opt.min_L;
     a.max_len = c->opt.max_walk_len;
     a.round = round;

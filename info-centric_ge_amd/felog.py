@@ -65,6 +65,18 @@ class Stats(C.Structure):
     ]
 
 
+BUDGET_FEATURES, BUDGET_PARAMS = 8, 4866
+
+
+class BudgetModel(C.Structure):
+    """felog_budget_model: MLP parameters, feature standardisation and box,
+    target standardisation."""
+    _fields_ = [("params", C.c_float * BUDGET_PARAMS),
+                ("feat_mean", C.c_float * 8), ("feat_std", C.c_float * 8),
+                ("feat_lo", C.c_float * 8), ("feat_hi", C.c_float * 8),
+                ("targ_mean", C.c_float * 2), ("targ_std", C.c_float * 2)]
+
+
 _lib = None
 
 
@@ -108,6 +120,15 @@ def load_library(path=LIB_PATH):
         "felog_stats_get": (C.c_int32, [P, C.POINTER(Stats)]),
         "felog_debug_sgns": (C.c_int32, [P, P, P, C.c_uint64, C.c_uint32, C.c_float, C.c_uint32,
                                          C.POINTER(C.c_uint32), P, P, P, P, P]),
+        "felog_budget_features": (C.c_int32, [P, P]),
+        "felog_budget_grad": (C.c_int32, [P, P, P, C.c_uint64, P, P, C.POINTER(C.c_double)]),
+        "felog_budget_train": (C.c_int32, [P, P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                                           P, P]),
+        "felog_budget_set": (C.c_int32, [P, C.POINTER(BudgetModel)]),
+        "felog_budget_copy_out": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32)]),
+        "felog_run_budget": (C.c_int32, [P]),
+        "felog_visits_reset": (C.c_int32, [P]),
+        "felog_visit_stats": (C.c_int32, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -211,6 +232,54 @@ class Felog:
 
     def run(self):
         self._check(self.lib.felog_run(self.ctx), "felog_run")
+
+    # -- learned walk budget ---------------------------------------------
+    def budget_features(self):
+        out = np.empty((self.V, BUDGET_FEATURES), np.float32)
+        self._check(self.lib.felog_budget_features(self.ctx, _ptr(out)), "felog_budget_features")
+        return out
+
+    def budget_grad(self, X, T, params):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        params = np.ascontiguousarray(params, np.float32)
+        grad = np.empty(BUDGET_PARAMS, np.float32)
+        loss = C.c_double()
+        self._check(self.lib.felog_budget_grad(self.ctx, _ptr(X), _ptr(T), len(X), _ptr(params), _ptr(grad),
+                                               C.byref(loss)), "felog_budget_grad")
+        return grad, loss.value
+
+    def budget_train(self, X, T, params, epochs, batch=4096, lr=1e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        p = np.ascontiguousarray(params, np.float32).copy()
+        losses = np.zeros(epochs, np.float64)
+        self._check(self.lib.felog_budget_train(self.ctx, _ptr(X), _ptr(T), len(X), epochs, batch, lr, seed,
+                                                _ptr(p), _ptr(losses)), "felog_budget_train")
+        return p, losses
+
+    def budget_set(self, model):
+        self._check(self.lib.felog_budget_set(self.ctx, None if model is None else C.byref(model)),
+                    "felog_budget_set")
+
+    def budget(self):
+        cnt = np.empty(self.V, np.uint16)
+        ln = np.empty(self.V, np.uint16)
+        r = C.c_uint32()
+        self._check(self.lib.felog_budget_copy_out(self.ctx, _ptr(cnt), _ptr(ln), C.byref(r)),
+                    "felog_budget_copy_out")
+        return cnt, ln, r.value
+
+    def run_budget(self):
+        self._check(self.lib.felog_run_budget(self.ctx), "felog_run_budget")
+
+    def visits_reset(self):
+        self._check(self.lib.felog_visits_reset(self.ctx), "felog_visits_reset")
+
+    def visit_stats(self):
+        out = np.zeros(4, np.float64)
+        self._check(self.lib.felog_visit_stats(self.ctx, _ptr(out)), "felog_visit_stats")
+        return {"kl": out[0], "entropy": out[1], "unvisited": int(out[2]), "visits": int(out[3])}
 
     # -- read-back -------------------------------------------------------
     def corpus(self):

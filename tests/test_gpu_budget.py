@@ -1,0 +1,196 @@
+"""GPU tests of the learned walk budget: features, the MLP's forward/backward
+kernels against the FP64 oracle, training, budgeted walks, the visit
+diagnostics, and what a trained network gains over the plain rule."""
+import time
+
+import numpy as np
+import pytest
+
+import budget
+import felog
+import graphgen
+import orc
+import walk_budget
+
+pytestmark = pytest.mark.gpu
+SEED = 0xB0D6E7
+
+
+def ctx(src, dst, w, n, **kw):
+    opts = dict(size=64, window=10, negative=5, sample=1e-3, min_L=20, min_R=2, seed=SEED)
+    opts.update(kw)
+    print(f"[case] V={n} m={len(src)} {opts}", flush=True)
+    f = felog.Felog(**opts)
+    f.set_edges(src, dst, w, n, make_undirected=True)
+    f.build()
+    return f
+
+
+@pytest.fixture(scope="module")
+def er():
+    src, dst, w = graphgen.erdos_renyi(2000, 8000, seed=5, weights="uniform")
+    f = ctx(src, dst, w, 2000)
+    yield f
+    f.close()
+
+
+def test_features_match_oracle(er):
+    got = er.budget_features()
+    rp, col, w = er.csr()
+    order, deg = er.vocab()
+    inv = (deg >= 5).astype(np.uint8)
+    inv[order[0]] = 1
+    ref = budget.features(rp, col, w, inv)
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-5), np.abs(got - ref).max(0)
+
+
+def test_mlp_gradient_matches_fp64():
+    """The HIP forward/backward over 1000 samples against FP64 backprop:
+    norm-wise relative error of the gradient below 1e-5 (f32 arithmetic,
+    per-workgroup sums added atomically)."""
+    src, dst, w = graphgen.erdos_renyi(100, 300, seed=1)
+    f = ctx(src, dst, w, 100)
+    try:
+        rng = np.random.default_rng(7)
+        p = budget.init_params(3)
+        p[-2:] = [0.1, -0.3]
+        X = rng.standard_normal((1000, 8)).astype(np.float32)
+        T = rng.standard_normal((1000, 2)).astype(np.float32)
+        g, loss = f.budget_grad(X, T, p)
+        lref, gref = budget.loss_and_grad(p, X, T)
+        err = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+        print(f"MLP gradient: relative error {err:.2e}, loss {loss:.6f} vs {lref:.6f}")
+        assert err < 1e-5
+        assert abs(loss - lref) < 1e-5 * lref
+    finally:
+        f.close()
+
+
+def test_mlp_training_fits_a_smooth_target():
+    """Adam on the GPU kernels fits a nonlinear function of the features;
+    the returned parameters' loss, recomputed in FP64, is what was trained."""
+    src, dst, w = graphgen.erdos_renyi(100, 300, seed=1)
+    f = ctx(src, dst, w, 100)
+    try:
+        rng = np.random.default_rng(11)
+        X = rng.standard_normal((20000, 8)).astype(np.float32)
+        T = np.stack([np.tanh(X[:, 0] + 0.5 * X[:, 1] * X[:, 2]), 0.3 * X[:, 3] ** 2 - X[:, 4]], 1).astype(np.float32)
+        p0 = budget.init_params(2)
+        l0, _ = budget.loss_and_grad(p0, X, T)
+        p, losses = f.budget_train(X, T, p0, epochs=40, batch=1024, lr=3e-3, seed=5)
+        l1, _ = budget.loss_and_grad(p, X, T)
+        print(f"MLP training: loss {l0:.4f} -> {l1:.4f}; per epoch {np.round(losses[::8], 4)}")
+        assert l1 < 0.05 * T.var(0).sum()
+        assert l1 < 0.1 * l0
+    finally:
+        f.close()
+
+
+def test_budgeted_walks_follow_the_budget(er):
+    """A constant budget (count 2, length 7): rounds 1-2 start every vertex
+    and every walk has 7 vertices; round 3 starts none.  Vertices outside
+    the model's feature box keep the plain rule (22-vertex walks here)."""
+    try:
+        er.budget_set(walk_budget.constant_model(2, 7))
+        cnt, ln, rounds = er.budget()
+        assert rounds == 2 and (cnt == 2).all() and (ln == 7).all()
+        for r in (1, 2):
+            c = er.walk_round(r)
+            off, tok, wid = er.corpus()
+            L = np.diff(off.astype(np.int64))
+            assert c["n_walks"] == 2000 and (L == 7).all()
+        assert er.walk_round(3)["n_walks"] == 0
+        # box on the first feature (ln(1 + degree)): vertices of degree >= 9
+        # fall back to the plain rule
+        feats = er.budget_features()
+        lo = np.full(8, -1e30)
+        hi = np.full(8, 1e30)
+        hi[0] = np.log1p(8.5)
+        er.budget_set(walk_budget.constant_model(2, 7, box=(lo, hi)))
+        cnt, ln, _ = er.budget()
+        plain = feats[:, 0] > hi[0]
+        assert plain.any() and (~plain).any()
+        assert (cnt[plain] == 0).all() and (cnt[~plain] == 2).all()
+        er.walk_round(1)
+        off, tok, _ = er.corpus()
+        L = np.diff(off.astype(np.int64))
+        start = tok[off[:-1].astype(np.int64)]
+        assert (L[~plain[start]] == 7).all()
+        assert (L[plain[start]] >= 22).all()  # the stop rule: at least min_L + 2
+    finally:
+        er.budget_set(None)
+
+
+def test_visit_stats_match_oracle(er):
+    er.visits_reset()
+    hist = np.zeros(2000, np.int64)
+    for r in (1, 2):
+        er.walk_round(r)
+        off, tok, _ = er.corpus()
+        hist += np.bincount(tok, minlength=2000)
+    got = er.visit_stats()
+    rp, _, _ = er.csr()
+    kl, ent, unv = budget.visit_stats(hist, np.diff(rp.astype(np.int64)))
+    print(f"visits: KL {got['kl']:.6f} entropy {got['entropy']:.6f}")
+    assert got["visits"] == hist.sum()
+    assert abs(got["kl"] - kl) < 1e-9 * max(1.0, abs(kl)) and abs(got["entropy"] - ent) < 1e-9 * ent
+    assert got["unvisited"] == unv
+
+
+def _auc(emb, pos, neg):
+    a, b = emb[pos[:, 0]], emb[pos[:, 1]]
+    cp = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-12)
+    a, b = emb[neg[:, 0]], emb[neg[:, 1]]
+    cn = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-12)
+    return orc.auc(cp.astype(np.float32), cn.astype(np.float32))
+
+
+def test_learned_budget_against_the_plain_rule():
+    """Train the network on the plain rule's budgets of one R-MAT graph and
+    run a second R-MAT graph with it: held-out AUC within 0.01 of the plain
+    rule's, the same corpus size within 10%, visit entropy within 2%, and
+    evaluation only for vertices outside the network's feature box."""
+    n = 20_000
+    a = graphgen.rmat(n, 100_000, seed=21)
+    fa = ctx(*a, n, alpha=0.025)
+    try:
+        cnt, length, _, _ = walk_budget.plain_budgets(fa, max_rounds=12)
+        X, T = walk_budget.training_set(fa.budget_features(), cnt, length)
+        model, losses = walk_budget.fit(fa, X, T, epochs=30, batch=1024, lr=2e-3)
+        print(f"budget training: {len(X)} samples, loss {losses[0]:.4f} -> {losses[-1]:.4f}; "
+              f"plain counts {np.bincount(cnt)[:8]} mean length {length[cnt > 0].mean():.2f}")
+    finally:
+        fa.close()
+
+    b = graphgen.rmat(n, 100_000, seed=22)
+    ts, td, tw, pos, neg = graphgen.lp_split(*b, n, frac=0.1, seed=3)
+    res = {}
+    for mode in ("plain", "learned"):
+        f = ctx(ts, td, tw, n, alpha=0.025)
+        try:
+            f.visits_reset()
+            t0 = time.perf_counter()
+            if mode == "plain":
+                cnt, length, corpora, _ = walk_budget.plain_budgets(f, max_rounds=12)
+                tokens = sum(int(o[-1]) for o, _ in corpora)
+            else:
+                f.budget_set(model)
+                bc, bl, rounds = f.budget()
+                print(f"learned budget: rounds {rounds}, counts {np.bincount(bc)[:8]}, "
+                      f"lengths {np.unique(bl)[:8]}, plain-rule fallback {(bc == 0).sum()}")
+                f.run_budget()
+                tokens = f.visit_stats()["visits"]
+            wall = time.perf_counter() - t0
+            st = f.stats()
+            emb, _ = f.embeddings()
+            vs = f.visit_stats()
+            res[mode] = dict(auc=_auc(emb, pos, neg), tokens=tokens, entropy=vs["entropy"], kl=vs["kl"],
+                             wall=wall, eval_ms=st["eval_ms_total"], walk_ms=st["walk_ms_total"])
+        finally:
+            f.close()
+    print({k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()})
+    p, l_ = res["plain"], res["learned"]
+    assert l_["auc"] >= p["auc"] - 0.01
+    assert abs(l_["tokens"] - p["tokens"]) <= 0.1 * p["tokens"]
+    assert abs(l_["entropy"] - p["entropy"]) <= 0.02 * p["entropy"]
+    assert l_["eval_ms"] <= p["eval_ms"]

@@ -370,3 +370,52 @@ def test_w2v_single_pair_update():
         a1[centre] = a1[centre] + g * a0[ctx]
         a0[ctx] = a0[ctx] + e
     assert np.allclose(s0, a0, atol=1e-6) and np.allclose(s1, a1, atol=1e-6)
+
+
+# ---- learned walk budget: FP64 network, features, visit diagnostics
+import budget  # noqa: E402
+
+
+def test_budget_mlp_gradient_finite_difference():
+    rng = np.random.default_rng(3)
+    p = budget.init_params(5).astype(np.float64)
+    p[budget.N_PARAMS - 2:] = [0.3, -0.2]
+    X = rng.standard_normal((40, 8))
+    T = rng.standard_normal((40, 2))
+    loss, g = budget.loss_and_grad(p, X, T)
+    assert abs(loss - ((budget.forward(p, X) - T) ** 2).sum() / 40) < 1e-12
+    eps = 1e-6
+    for i in rng.choice(budget.N_PARAMS, 300, replace=False):
+        q = p.copy()
+        q[i] += eps
+        lp = ((budget.forward(q, X) - T) ** 2).sum() / 40
+        q[i] -= 2 * eps
+        lm = ((budget.forward(q, X) - T) ** 2).sum() / 40
+        assert abs((lp - lm) / (2 * eps) - g[i]) < 1e-6 * max(1.0, abs(g[i]))
+
+
+def test_budget_features_by_hand():
+    # star 0-{1,2,3} with weights 1,2,1 and an edge 3-4 (w 4)
+    src = np.array([0, 0, 0, 3], np.uint32)
+    dst = np.array([1, 2, 3, 4], np.uint32)
+    w = np.array([1, 2, 1, 4], np.float32)
+    rp, col, wo = orc.build_csr(5, src, dst, w, True)
+    f = budget.features(rp, col, wo, np.array([1, 0, 0, 1, 0]))
+    ws = np.array([1, 2, 1]) / 4.0
+    assert np.allclose(f[0], [math.log(4), math.log(5), 2, 4 / 3, math.log(1 + 1 + 1 + 2), math.log(3),
+                              -(ws * np.log(ws)).sum() / math.log(3), 1])
+    # vertex 3: neighbours 0 (deg 3) and 4 (deg 1), weights 1 and 4
+    ws = np.array([1, 4]) / 5.0
+    assert np.allclose(f[3], [math.log(3), math.log(6), 4, 2.5, math.log(5), math.log(4),
+                              -(ws * np.log(ws)).sum() / math.log(2), 1])
+    assert np.allclose(f[4], [math.log(2), math.log(5), 4, 4, math.log(3), math.log(3), 0, 0])
+
+
+def test_visit_stats_closed_form():
+    deg = np.array([2, 1, 1, 0])
+    visits = np.array([1, 1, 0, 2])
+    kl, ent, unv = budget.visit_stats(visits, deg)
+    # p = (1/2, 1/4, 1/4, 0), q = (1/4, 1/4, 0, 1/2)
+    assert abs(kl - (0.5 * math.log(2) + 0.25 * math.log(1))) < 1e-15
+    assert abs(ent - (2 * 0.25 * math.log(4) + 0.5 * math.log(2))) < 1e-15
+    assert unv == 1
