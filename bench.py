@@ -3,18 +3,24 @@
 
 One step = one round of the path: a walk of every walker over the resident
 graph (persistent walk kernel + corpus compaction), then skip-gram training
-over that round's corpus (SGNS kernel).  The graph, alias tables and both
+over that round's corpus (SGNS kernels).  The graph, alias tables and both
 embedding tables are built in HBM before the timed region; nothing crosses
 PCIe inside it.
 
-Workload (default): R-MAT (a=0.57, b=0.19, c=0.19), 1.1M vertices, ~3M
-undirected edges, weights U(0,1], info-centric walks (min_L=20), FP32
-HogBatch SGNS with d=128, window 10, negative 5, batch 21.
+Workload (default): Chung-Lu power law (exponent 2.2), 2.24M vertices,
+~35M undirected edges, weights U(0,1], info-centric walks (min_L=20), plain
+FP32 HogBatch SGNS with d=128, window 10, negative 5, batch 21.
 
 Prints one JSON line.  `value` is corpus tokens walked and trained per
-second of the whole round; walk steps/s and SGNS words/s are reported beside
-it.  The roofline is that of the kernel that takes the most time, from HIP
-events recorded on the library's stream around each launch.
+second of the whole round (plain FP32 form); walk steps/s and SGNS words/s
+are reported beside it.  The roofline is that of the kernel that takes the
+most time, from HIP events recorded on the library's stream around each
+launch.  The `e2e` object holds the end-to-end comparison the metric names:
+on the same graph with 10% of its edges held out, the plain FP32 form run
+to the plain rule's stop (felog_run) against the BF16 form with the learned
+walk budget (felog_run_budget; network trained on the plain rule's budgets
+of the 10k Erdos-Renyi and 1.1M R-MAT graphs), with held-out AUC, corpus
+size and visit entropy of both.
 """
 from __future__ import annotations
 
@@ -38,8 +44,8 @@ FP32_PEAK_TFS = 157.3
 
 WORKLOADS = {
     # name: (generator, vertices, undirected edges, weight law)
-    "rmat-1.1M": ("rmat", 1_100_000, 3_000_000, "uniform"),
     "chunglu-2.24M": ("chung_lu", 2_240_000, 35_000_000, "uniform"),
+    "rmat-1.1M": ("rmat", 1_100_000, 3_000_000, "uniform"),
     "er-10k": ("erdos_renyi", 10_000, 50_000, "uniform"),
 }
 
@@ -60,13 +66,89 @@ def measured_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 counter summary of
     this workload (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy
     calibration), or {} when there is none for it."""
-    tag = {"rmat-1.1M": "rmat1.1M"}.get(args.workload)
+    tag = {"rmat-1.1M": "rmat1.1M", "chunglu-2.24M": "chunglu2.24M"}.get(args.workload)
     path = os.path.join(ROOT, "profiles", f"traffic_{tag}_{'f32' if args.precision == 'fp32' else 'bf16'}.json")
     if tag is None or args.dim != 128 or not os.path.exists(path):
         return {}
     with open(path) as fh:
         kernels = json.load(fh)["kernels"]
     return {name: round(v["hbm_bytes"]) for name, v in kernels.items()}
+
+
+def _auc(emb, pos, neg, orc):
+    def cos(pairs):
+        a, b = emb[pairs[:, 0]], emb[pairs[:, 1]]
+        return (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-12)
+    return orc.auc(cos(pos).astype(np.float32), cos(neg).astype(np.float32))
+
+
+def train_budget_model(args):
+    """The learned walk budget: plain-rule budgets of the 10k Erdos-Renyi
+    (alpha 0.01) and 1.1M R-MAT graphs, then Adam on the library's kernels."""
+    import walk_budget
+    Xs, Ts, info = [], [], {}
+    t0 = time.perf_counter()
+    for name, (gen, n, m), kw in (("er-10k", ("erdos_renyi", 10_000, 50_000), dict(alpha=0.01)),
+                                  ("rmat-1.1M", ("rmat", 1_100_000, 3_000_000), {})):
+        src, dst, w = getattr(graphgen, gen)(n, m, seed=args.seed if gen == "rmat" else 1, weights="uniform")
+        f = felog.Felog(size=args.dim, window=10, negative=5, sample=1e-3, min_L=20, min_R=2, seed=args.seed, **kw)
+        f.set_edges(src, dst, w, n, make_undirected=True)
+        f.build()
+        cnt, length, _, _ = walk_budget.plain_budgets(f, max_rounds=12)
+        X, T = walk_budget.training_set(f.budget_features(), cnt, length)
+        Xs.append(X)
+        Ts.append(T)
+        info[name] = {"count_histogram": np.bincount(cnt).tolist()[:8], "mean_length": float(length[cnt > 0].mean())}
+        if name == "rmat-1.1M":
+            model, losses = walk_budget.fit(f, np.concatenate(Xs), np.concatenate(Ts), epochs=20, batch=4096,
+                                            lr=2e-3, seed=args.seed)
+        f.close()
+    info["samples"] = int(sum(len(x) for x in Xs))
+    info["loss_first_last_epoch"] = [float(losses[0]), float(losses[-1])]
+    info["seconds"] = time.perf_counter() - t0
+    return model, info
+
+
+def end_to_end(src, dst, w, n, args, model):
+    """Plain FP32 run (felog_run) against BF16 + learned budget
+    (felog_run_budget) on the graph with 10% of its edges held out."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
+    out = {}
+    for form, prec, budgeted in (("plain_fp32", felog.FP32, False), ("bf16_surrogate", felog.BF16, True)):
+        f = felog.Felog(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20, min_R=2,
+                        precision=prec, seed=args.seed)
+        f.set_edges(ts, td, tw, n, make_undirected=True)
+        f.build()
+        f.visits_reset()
+        extra = {}
+        if budgeted:
+            f.budget_set(model)
+            cnt, ln, rounds = f.budget()
+            extra = {"budgeted_vertices": int((cnt > 0).sum()), "plain_rule_fallback": int((cnt == 0).sum()),
+                     "max_count": int(rounds)}
+        t0 = time.perf_counter()
+        if budgeted:
+            f.run_budget()
+        else:
+            f.run()
+        wall = time.perf_counter() - t0
+        st = f.stats()
+        vs = f.visit_stats()
+        emb, _ = f.embeddings()
+        out[form] = dict(wall_ms=wall * 1e3, rounds=st["rounds_done"], tokens=vs["visits"],
+                         tokens_per_s=vs["visits"] / wall, auc=_auc(emb, pos, neg, orc), visit_kl=vs["kl"],
+                         visit_entropy=vs["entropy"], walk_ms=st["walk_ms_total"], train_ms=st["train_ms_total"],
+                         eval_ms=st["eval_ms_total"], **extra)
+        f.close()
+    p, b = out["plain_fp32"], out["bf16_surrogate"]
+    out["throughput_ratio"] = p["wall_ms"] / b["wall_ms"]
+    out["auc_gap"] = b["auc"] - p["auc"]
+    out["definition"] = ("throughput_ratio = wall time of the plain FP32 run / wall time of the BF16 + "
+                         "learned-budget run, each from the first walk to its stop on the same graph "
+                         "(vertices embedded per second); auc_gap = held-out AUC difference")
+    return out
 
 
 def cpu_baseline(src, dst, w, n, args, seed, threads):
@@ -107,12 +189,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat-1.1M", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="chunglu-2.24M", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-walkers", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end plain vs BF16 + budget runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -208,6 +291,13 @@ def main():
         "cpu_baseline": None,
     }
     f.close()
+    if not args.no_e2e:
+        t0 = time.perf_counter()
+        model, info = train_budget_model(args)
+        print(f"[bench] budget model trained ({info['seconds']:.1f}s)", file=sys.stderr, flush=True)
+        out["e2e"] = end_to_end(src, dst, w, n, args, model)
+        out["e2e"]["budget_model"] = info
+        print(f"[bench] end-to-end comparison ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(src, dst, w, n, args, args.seed, threads)

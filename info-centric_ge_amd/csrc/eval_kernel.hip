@@ -38,13 +38,15 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                                               uint8_t *__restrict__ flags, float *__restrict__ scores,
                                               const T *__restrict__ syn0, uint32_t V, uint32_t dp,
                                               uint32_t d, uint32_t round, uint64_t seed,
+                                              const uint16_t *__restrict__ budget_cnt,
                                               unsigned long long *counters) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long n_eval = 0, n_conv = 0;
     for (uint32_t v = wave; v < V; v += nwaves) {
-        if (!in_vocab[v] || flags[v]) continue;
+        // vertices with a learned walk budget are not judged by convergence
+        if (!in_vocab[v] || flags[v] || (budget_cnt && budget_cnt[v])) continue;
         const uint64_t b = row_ptr[v];
         const uint32_t deg = (uint32_t)(row_ptr[v + 1] - b);
         float score = 0.f;
@@ -113,16 +115,17 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
 hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms) {
     HIP_TRY(hipMemsetAsync(c->d_counters + C_EVALNUM, 0, 16, c->stream));
     const unsigned blocks = c->cu_count * 8;
+    const uint16_t *bcnt = c->budget_on ? c->d_budget_cnt : nullptr;
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     if (c->opt.precision == FELOG_BF16)
         k_eval<uint16_t><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
                                                        c->d_scores, (const uint16_t *)c->d_syn0, c->V, c->dp,
-                                                       c->d, round, c->opt.seed,
+                                                       c->d, round, c->opt.seed, bcnt,
                                                        (unsigned long long *)c->d_counters);
     else
         k_eval<float><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
                                                     c->d_scores, (const float *)c->d_syn0, c->V, c->dp, c->d,
-                                                    round, c->opt.seed, (unsigned long long *)c->d_counters);
+                                                    round, c->opt.seed, bcnt, (unsigned long long *)c->d_counters);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));

@@ -571,6 +571,13 @@ felog_status felog_budget_set(felog_ctx *c, const felog_budget_model *model) {
     for (uint16_t x : cnt) mx = x > mx ? x : mx;
     c->budget_rounds = mx;
     c->budget_on = true;
+    // the plain rule's stop ratio now runs over the plain-rule vertices only:
+    // its first denominator is their vocabulary count (src/word2vec.cu:1980)
+    std::vector<uint8_t> inv(c->V);
+    TRY(c, hipMemcpy(inv.data(), c->d_in_vocab, c->V, hipMemcpyDeviceToHost));
+    uint64_t plain_vocab = 0;
+    for (uint32_t v = 0; v < c->V; ++v) plain_vocab += (inv[v] && cnt[v] == 0) ? 1 : 0;
+    c->last_eva_num = (uint32_t)plain_vocab;
     return FELOG_OK;
 }
 
