@@ -135,6 +135,8 @@ typedef struct felog_stats {
     double walk_ms_total, train_ms_total, eval_ms_total;  /* all rounds     */
     double sgns_prep_ms;             /* last round: SGNS tile preparation   */
     double sgns_tile_ms;             /* last round: SGNS tile kernel        */
+    double graph_ms_total;           /* captured rounds (felog_run_graph),
+                                        HIP events around each replay       */
 } felog_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -178,8 +180,20 @@ felog_status felog_evaluate(felog_ctx *ctx, uint32_t *n_unconverged,
                             int *stop);
 /* Rounds until the stop rule fires or max_rounds is reached. */
 felog_status felog_run(felog_ctx *ctx);
+/* felog_run (or, with a walk budget installed, felog_run_budget) with each
+ * round - walker select, walks, corpus, SGNS pass, evaluation and the stop
+ * decision - captured once as a hipGraph and replayed; the round number,
+ * learning rate, walker count and stop state live in device memory, and the
+ * host waits once per round.  Same walks, corpus and stop decisions as the
+ * eager loop (src/word2vec.cu:1955-2082 drives the same sequence).  Runs at
+ * most max_rounds more rounds (0: until the stop rule or the options'
+ * max_rounds); a later call continues from the rounds done. */
+felog_status felog_run_graph(felog_ctx *ctx, uint32_t max_rounds);
 
 /* ---- read-back (caller-allocated host buffers) ------------------------- */
+/* Descriptor of the context's current corpus: the last walk round's, also
+ * after felog_run and felog_run_graph. */
+felog_status felog_corpus_get(felog_ctx *ctx, felog_corpus *out);
 felog_status felog_corpus_copy_out(felog_ctx *ctx, uint64_t *offsets,
                                    uint32_t *tokens, uint32_t *walker_ids);
 felog_status felog_csr_copy_out(felog_ctx *ctx, uint64_t *row_ptr,

@@ -264,7 +264,9 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F
 // Walkers of round r: walker i starts at i % V; budgeted vertices walk in
 // rounds 1..count, the others while not flagged.
 __global__ void k_walk_ok_budget(const uint8_t *__restrict__ flags, const uint16_t *__restrict__ cnt,
-                                 uint32_t walkers, uint32_t V, uint32_t round, uint8_t *__restrict__ ok) {
+                                 uint32_t walkers, uint32_t V, const RoundState *__restrict__ rs,
+                                 uint32_t *__restrict__ ok) {
+    const uint32_t round = rs->round;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < walkers; i += stride) {
         const uint32_t v = i % V;
@@ -272,7 +274,10 @@ __global__ void k_walk_ok_budget(const uint8_t *__restrict__ flags, const uint16
     }
 }
 
-__global__ void k_visits(const uint32_t *__restrict__ tok, uint64_t n, uint32_t *__restrict__ visits) {
+// n_dev (when set) holds the token count instead of n.
+__global__ void k_visits(const uint32_t *__restrict__ tok, uint64_t n, const uint64_t *__restrict__ n_dev,
+                         uint32_t *__restrict__ visits) {
+    if (n_dev) n = *n_dev;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         atomicAdd(&visits[tok[i]], 1u);
@@ -349,19 +354,25 @@ hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_
     return hipGetLastError();
 }
 
-hipError_t budget_walk_ok(felog_ctx *c, uint32_t round) {
+hipError_t budget_walk_ok(felog_ctx *c, uint32_t *d_ok) {
     k_walk_ok_budget<<<grid_for(c->walkers, 256, 65535), 256, 0, c->stream>>>(c->d_flags, c->d_budget_cnt,
-                                                                             c->walkers, c->V, round,
-                                                                             c->d_walk_ok);
+                                                                             c->walkers, c->V, c->d_rs,
+                                                                             d_ok);
     return hipGetLastError();
 }
 
 hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok) {
     if (n_tok == 0) return hipSuccess;
-    k_visits<<<grid_for(n_tok, 256, 1u << 16), 256, 0, c->stream>>>(d_tok, n_tok, c->d_visits);
+    k_visits<<<grid_for(n_tok, 256, 1u << 16), 256, 0, c->stream>>>(d_tok, n_tok, nullptr, c->d_visits);
     HIP_TRY(hipGetLastError());
     c->visits_total += n_tok;
     return hipSuccess;
+}
+
+// The caller adds the round's token count to visits_total once it is known.
+hipError_t add_round_visits(felog_ctx *c) {
+    k_visits<<<c->cu_count * 8, 256, 0, c->stream>>>(c->d_tok, 0, &c->d_rs->n_tokens, c->d_visits);
+    return hipGetLastError();
 }
 
 hipError_t visit_stats(felog_ctx *c, double *d_out) {

@@ -37,9 +37,12 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                                               const uint8_t *__restrict__ in_vocab,
                                               uint8_t *__restrict__ flags, float *__restrict__ scores,
                                               const T *__restrict__ syn0, uint32_t V, uint32_t dp,
-                                              uint32_t d, uint32_t round, uint64_t seed,
-                                              const uint16_t *__restrict__ budget_cnt,
+                                              uint32_t d, const RoundState *__restrict__ rs, int gated,
+                                              uint64_t seed, const uint16_t *__restrict__ budget_cnt,
                                               unsigned long long *counters) {
+    const RoundState s = *rs;
+    if (gated && !round_evaluates(s)) return;
+    const uint32_t round = s.round;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -112,21 +115,27 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
         if (e_ != hipSuccess) return e_;        \
     } while (0)
 
-hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms) {
-    HIP_TRY(hipMemsetAsync(c->d_counters + C_EVALNUM, 0, 16, c->stream));
+hipError_t eval_enqueue(felog_ctx *c, bool gated) {
     const unsigned blocks = c->cu_count * 8;
     const uint16_t *bcnt = c->budget_on ? c->d_budget_cnt : nullptr;
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
     if (c->opt.precision == FELOG_BF16)
         k_eval<uint16_t><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
                                                        c->d_scores, (const uint16_t *)c->d_syn0, c->V, c->dp,
-                                                       c->d, round, c->opt.seed, bcnt,
+                                                       c->d, c->d_rs, gated ? 1 : 0, c->opt.seed, bcnt,
                                                        (unsigned long long *)c->d_counters);
     else
         k_eval<float><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
                                                     c->d_scores, (const float *)c->d_syn0, c->V, c->dp, c->d,
-                                                    round, c->opt.seed, bcnt, (unsigned long long *)c->d_counters);
-    HIP_TRY(hipGetLastError());
+                                                    c->d_rs, gated ? 1 : 0, c->opt.seed, bcnt,
+                                                    (unsigned long long *)c->d_counters);
+    return hipGetLastError();
+}
+
+// Eager evaluation of the round the round state holds.
+hipError_t run_eval(felog_ctx *c, float *ms) {
+    HIP_TRY(hipMemsetAsync(c->d_counters + C_EVALNUM, 0, 16, c->stream));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(eval_enqueue(c, false));
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
     HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));

@@ -44,28 +44,38 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Learning rate of round r (1-based): ExponentialDecayLR(base, 0.1) with one
-// get_lr() call per round (include/lr_scheduler.hpp:23-31,
-// src/word2vec.cu:1958,2001).  The reference always uses base 0.025; the
-// default here honours -alpha as the base.  A tile sums all of its pair
-// gradients at one point (the reference's sequential per-pair updates let
-// the sigmoid saturate between pairs), so the default base is 0.0025: on the
-// 1.1M R-MAT and 2.24M Chung-Lu graphs 0.025 and 0.005 drive held-out AUC
-// down after one or two rounds, 0.0025 keeps it at or above word2vec's.
-float round_alpha(const felog_options &o, uint32_t round) {
-    const float base = o.compat_lr ? 0.025f : o.alpha;
-    const float decay = 0.1f;
-    return base * std::exp(-decay * (float)(round - 1));
+}  // namespace
+
+namespace felog {
+// Base of the learning-rate schedule (the rate of round r is
+// base * e^{-0.1 (r-1)}, computed on the device).  The reference always uses
+// base 0.025; the default here honours -alpha as the base.  A tile sums all
+// of its pair gradients at one point (the reference's sequential per-pair
+// updates let the sigmoid saturate between pairs), so the default base is
+// 0.0025: on the 1.1M R-MAT and 2.24M Chung-Lu graphs 0.025 and 0.005 drive
+// held-out AUC down after one or two rounds, 0.0025 keeps it at or above
+// word2vec's.
+float alpha_base(const felog_options &o) { return o.compat_lr ? 0.025f : o.alpha; }
+
+void drop_round_graph(felog_ctx *c) {
+    if (c->round_exec) (void)hipGraphExecDestroy(c->round_exec);
+    c->round_exec = nullptr;
 }
+}  // namespace felog
+
+namespace {
 
 void free_all(felog_ctx *c) {
     void *ptrs[] = {c->d_esrc, c->d_edst, c->d_ew, c->d_row_ptr, c->d_col, c->d_w, c->d_vrec, c->d_erec,
                     c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni_thr,
                     c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
-                    c->d_scores, c->d_active, c->d_walk_ok, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
+                    c->d_scores, c->d_active, c->d_sel, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
                     c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats};
     for (void *p : ptrs) dfree(p);
+    drop_round_graph(c);
+    dfree(c->d_rs);
+    if (c->h_rs) (void)hipHostFree(c->h_rs);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -141,6 +151,13 @@ felog_status felog_create(felog_ctx **out, const felog_options *opt) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev2);
     if (e == hipSuccess) e = hipMalloc(&c->d_counters, kNumCounters * 8);
     if (e == hipSuccess) e = hipHostMalloc(&c->h_counters, kNumCounters * 8, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&c->d_rs, sizeof(RoundState));
+    if (e == hipSuccess) e = hipHostMalloc(&c->h_rs, sizeof(RoundState), hipHostMallocDefault);
+    if (e == hipSuccess) {
+        std::memset(c->h_rs, 0, sizeof(RoundState));
+        c->h_rs->eval_from = o.min_R;
+        e = hipMemcpy(c->d_rs, c->h_rs, sizeof(RoundState), hipMemcpyHostToDevice);
+    }
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     if (e != hipSuccess) {
@@ -256,12 +273,13 @@ felog_status felog_build(felog_ctx *c) {
     c->slot = slot;
     const uint64_t W = c->walkers;
     TRY(c, hipMalloc(&c->d_active, W * 4));
-    TRY(c, hipMalloc(&c->d_walk_ok, W));
+    TRY(c, hipMalloc(&c->d_sel, W * 4));
     TRY(c, hipMalloc(&c->d_slots, W * slot * 4));
     TRY(c, hipMalloc(&c->d_lens, W * 4));
     TRY(c, hipMalloc(&c->d_ovf_base, W * 8));
     c->ovf_cap = c->opt.max_walk_len > slot ? c->opt.overflow_capacity : 0;
     TRY(c, hipMalloc(&c->d_ovf, (c->ovf_cap ? c->ovf_cap : 1) * 4));
+    TRY(c, round_buffers(c));
     c->last_eva_num = (uint32_t)c->vocab_size;
     c->n_unconverged = (uint32_t)c->vocab_size;
     c->st.build_ms = now_ms() - t0;
@@ -278,12 +296,13 @@ felog_status felog_walk_round(felog_ctx *c, uint32_t round, felog_corpus *out) {
     if (round == 0) return fail(c, FELOG_ERR_ARG, "rounds are numbered from 1");
     float wms = 0.f, cms = 0.f;
     c->corpus = Corpus();
-    TRY(c, run_walks(c, round, 0, &wms));
+    TRY(c, round_set(c, round, -1.f, ~0u));
+    TRY(c, run_walks(c, &wms));
     const uint64_t fail_n = c->h_counters[C_OVF_FAIL];
     const uint64_t steps = c->h_counters[C_STEPS];
     const uint64_t maxlen = c->h_counters[C_MAXLEN];
     c->st.overflow_used = c->h_counters[C_OVF_NEXT];
-    TRY(c, compact_corpus(c, (uint32_t)c->corpus.n_walks, &cms));
+    TRY(c, compact_corpus(c, &cms));
     if (c->d_visits) TRY(c, add_visits(c, c->d_tok, c->corpus.n_tokens));
     c->corpus.round = round;
     c->corpus.n_steps = steps;
@@ -308,10 +327,13 @@ felog_status felog_walk_round(felog_ctx *c, uint32_t round, felog_corpus *out) {
     return FELOG_OK;
 }
 
+// alpha < 0: the schedule's rate for the round
 static felog_status train_device_corpus(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n,
                                         uint64_t n_tok, uint32_t round, float alpha) {
+    if (n >= 0xFFFFFFFFull) return fail(c, FELOG_ERR_ARG, "a corpus holds fewer than 2^32 - 1 sentences");
     float ms[3] = {0.f, 0.f, 0.f};
-    TRY(c, run_sgns(c, d_off, d_tok, n, n_tok, round, alpha, nullptr, ms));
+    TRY(c, round_set(c, round, alpha, (uint32_t)n));
+    TRY(c, run_sgns(c, d_off, d_tok, n, n_tok, nullptr, ms));
     c->st.train_words = c->h_counters[C_WORDS];
     c->st.train_kept = c->h_counters[C_KEPT];
     c->st.train_pairs = c->h_counters[C_PAIRS];
@@ -333,8 +355,7 @@ felog_status felog_train_round(felog_ctx *c, const felog_corpus *corpus) {
     if (corpus && corpus->round != c->corpus.round)
         return fail(c, FELOG_ERR_ARG, "corpus handle is not the context's current corpus");
     const uint32_t round = c->corpus.round;
-    s = train_device_corpus(c, c->d_off, c->d_tok, c->corpus.n_walks, c->corpus.n_tokens, round,
-                            round_alpha(c->opt, round));
+    s = train_device_corpus(c, c->d_off, c->d_tok, c->corpus.n_walks, c->corpus.n_tokens, round, -1.f);
     if (s) return s;
     c->rounds_done = round;
     return FELOG_OK;
@@ -360,7 +381,7 @@ felog_status felog_train_corpus(felog_ctx *c, const uint64_t *offsets, const uin
     TRY(c, hipMalloc(&d_tok, (nt ? nt : 1) * 4));
     TRY(c, hipMemcpy(d_off, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
     if (nt) TRY(c, hipMemcpy(d_tok, tokens, nt * 4, hipMemcpyHostToDevice));
-    s = train_device_corpus(c, d_off, d_tok, n, nt, round, round_alpha(c->opt, round));
+    s = train_device_corpus(c, d_off, d_tok, n, nt, round, -1.f);
     dfree(d_off);
     dfree(d_tok);
     return s;
@@ -374,6 +395,7 @@ felog_status felog_debug_sgns(felog_ctx *c, const uint64_t *offsets, const uint3
     if (s) return s;
     if (!offsets || !n_groups || !groups_out || !rows_out || !logits_out || !dv_out || !du_out || max_groups == 0)
         return fail(c, FELOG_ERR_ARG, "null debug buffer");
+    if (n >= 0xFFFFFFFFull) return fail(c, FELOG_ERR_ARG, "a corpus holds fewer than 2^32 - 1 sentences");
     const uint64_t nt = offsets[n];
     for (uint64_t i = 0; i < nt; ++i)
         if (tokens[i] >= c->V) return fail(c, FELOG_ERR_ARG, "token id >= v_num");
@@ -396,7 +418,8 @@ felog_status felog_debug_sgns(felog_ctx *c, const uint64_t *offsets, const uint3
     TRY(c, hipMemset(dbg.dv, 0, G * 64 * c->d * 4));
     TRY(c, hipMemset(dbg.du, 0, G * 32 * c->d * 4));
     float ms[3];
-    TRY(c, run_sgns(c, d_off, d_tok, n, nt, round, alpha, &dbg, ms));
+    TRY(c, round_set(c, round, alpha, (uint32_t)n));
+    TRY(c, run_sgns(c, d_off, d_tok, n, nt, &dbg, ms));
     // slots: one per sentence (empty when it has no centre) plus extra tiles
     const uint64_t ng = n + c->h_counters[C_TILES];
     *n_groups = (uint32_t)(ng < G ? ng : G);
@@ -416,7 +439,8 @@ felog_status felog_evaluate(felog_ctx *c, uint32_t *n_unconverged, int *stop) {
     felog_status s = need_built(c);
     if (s) return s;
     float ms = 0.f;
-    TRY(c, run_eval(c, c->rounds_done, &ms));
+    TRY(c, round_set(c, c->rounds_done, -1.f, ~0u));
+    TRY(c, run_eval(c, &ms));
     const uint32_t eva = (uint32_t)c->h_counters[C_EVALNUM];
     // src/word2vec.cu:2073-2078: stop once the unconverged count shrinks by
     // less than 30% in a round.
@@ -568,8 +592,13 @@ felog_status felog_budget_set(felog_ctx *c, const felog_budget_model *model) {
     TRY(c, hipMemcpyAsync(cnt.data(), c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost, c->stream));
     TRY(c, hipStreamSynchronize(c->stream));
     uint32_t mx = 0;
-    for (uint16_t x : cnt) mx = x > mx ? x : mx;
+    bool plain_left = false;
+    for (uint16_t x : cnt) {
+        mx = x > mx ? x : mx;
+        plain_left = plain_left || x == 0;
+    }
     c->budget_rounds = mx;
+    c->budget_plain_left = plain_left;
     c->budget_on = true;
     // the plain rule's stop ratio now runs over the plain-rule vertices only:
     // its first denominator is their vocabulary count (src/word2vec.cu:1980)
@@ -596,11 +625,7 @@ felog_status felog_run_budget(felog_ctx *c) {
     if (s) return s;
     if (!c->budget_on) return fail(c, FELOG_ERR_STATE, "no budget installed");
     // plain-rule vertices: those with count 0
-    std::vector<uint16_t> cnt(c->V);
-    TRY(c, hipMemcpy(cnt.data(), c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
-    bool plain_left = false;
-    for (uint16_t x : cnt) plain_left = plain_left || x == 0;
-    bool plain_stopped = !plain_left;
+    bool plain_stopped = !c->budget_plain_left;
     for (uint32_t r = c->rounds_done + 1;; ++r) {
         if (c->opt.max_rounds && r > c->opt.max_rounds) break;
         if (r > c->budget_rounds && plain_stopped) break;
@@ -613,6 +638,115 @@ felog_status felog_run_budget(felog_ctx *c) {
             int stop = 0;
             if ((s = felog_evaluate(c, &nu, &stop))) return s;
             if (stop) plain_stopped = true;
+        }
+    }
+    return FELOG_OK;
+}
+
+// ---- captured rounds -------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+// One round as a graph: round advance (counters cleared by a kernel),
+// walker select and walks, corpus, visits, SGNS pass, gated evaluation and
+// the stop decision.  Every launch reads its per-round values from the
+// device round state; no memset or host copy is recorded.
+hipError_t capture_round(felog_ctx *c, bool visits) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return e;
+    e = round_next(c);
+    if (e == hipSuccess) e = walk_enqueue(c);
+    if (e == hipSuccess) e = compact_enqueue(c);
+    if (e == hipSuccess && visits) e = add_round_visits(c);
+    if (e == hipSuccess) e = sgns_enqueue(c, c->d_off, c->d_tok, nullptr, nullptr);
+    if (e == hipSuccess) e = eval_enqueue(c, true);
+    if (e == hipSuccess) e = round_end(c);
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&c->round_exec, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+felog_status felog_run_graph(felog_ctx *c, uint32_t max_rounds) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    const bool budget = c->budget_on;
+    const bool visits = c->d_visits != nullptr;
+    // descriptors for the largest corpus a round can write
+    TRY(c, sgns_reserve(c, c->walkers, c->tok_cap));
+    if (c->round_exec && (c->round_exec_budget != budget || c->round_exec_visits != visits)) drop_round_graph(c);
+    if (!c->round_exec) {
+        TRY(c, capture_round(c, visits));
+        c->round_exec_budget = budget;
+        c->round_exec_visits = visits;
+    }
+    bool plain_stopped = budget && !c->budget_plain_left;
+    RoundState &h = *c->h_rs;
+    std::memset(&h, 0, sizeof(h));
+    h.round = c->rounds_done;
+    h.stop = plain_stopped ? 1 : 0;
+    h.last_eva = c->last_eva_num;
+    h.eval_from = c->opt.min_R;
+    h.budget = budget ? 1 : 0;
+    TRY(c, hipMemcpyAsync(c->d_rs, c->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, c->stream));
+    const uint32_t first = c->rounds_done + 1;
+    for (uint32_t r = first;; ++r) {
+        if (c->opt.max_rounds && r > c->opt.max_rounds) break;
+        if (max_rounds && r - first >= max_rounds) break;
+        if (budget && r > c->budget_rounds && plain_stopped) break;
+        TRY(c, hipEventRecord(c->ev0, c->stream));
+        TRY(c, hipGraphLaunch(c->round_exec, c->stream));
+        TRY(c, hipEventRecord(c->ev1, c->stream));
+        TRY(c, hipMemcpyAsync(c->h_rs, c->d_rs, sizeof(RoundState), hipMemcpyDeviceToHost, c->stream));
+        TRY(c, hipMemcpyAsync(c->h_counters, c->d_counters, kNumCounters * 8, hipMemcpyDeviceToHost, c->stream));
+        TRY(c, hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        const uint64_t *k = c->h_counters;
+        c->st.graph_ms_total += ms;
+        c->corpus = Corpus();
+        c->corpus.round = r;
+        c->corpus.n_walks = h.n_active;
+        c->corpus.n_tokens = h.n_tokens;
+        c->corpus.n_steps = k[C_STEPS];
+        c->corpus.max_len = (uint32_t)k[C_MAXLEN];
+        c->corpus.valid = true;
+        c->corpus.from_walk = true;
+        c->st.walk_steps = k[C_STEPS];
+        c->st.walk_tokens = h.n_tokens;
+        c->st.overflow_used = k[C_OVF_NEXT];
+        c->st.train_words = k[C_WORDS];
+        c->st.train_kept = k[C_KEPT];
+        c->st.train_pairs = k[C_PAIRS];
+        c->st.train_groups = k[C_GROUPS];
+        if (visits) c->visits_total += h.n_tokens;
+        if (k[C_OVF_FAIL])
+            return fail(c, FELOG_ERR_CAPACITY,
+                        std::to_string(k[C_OVF_FAIL]) + " walks outgrew the overflow pool; raise overflow_capacity");
+        if (h.n_active + k[C_TILES] > c->tiles_cap)
+            return fail(c, FELOG_ERR_INVARIANT, "SGNS tile descriptors exceeded their bound");
+        // the budget loop stops before training a round without walkers
+        if (budget && h.n_active == 0) break;
+        c->rounds_done = r;
+        if (h.evaluated) {
+            c->last_eva_num = h.last_eva;
+            c->n_unconverged = h.last_eva;
+            c->st.n_unconverged = h.last_eva;
+            c->stop = h.stop != 0;
+        }
+        if (budget) {
+            plain_stopped = h.stop != 0;
+        } else {
+            if (h.stop) break;
+            if (h.n_active == 0) break;
         }
     }
     return FELOG_OK;
@@ -642,6 +776,19 @@ felog_status felog_visit_stats(felog_ctx *c, double out[4]) {
     out[1] = h[1];
     out[2] = h[2];
     out[3] = (double)c->visits_total;
+    return FELOG_OK;
+}
+
+felog_status felog_corpus_get(felog_ctx *c, felog_corpus *out) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!out) return fail(c, FELOG_ERR_ARG, "null output");
+    if (!c->corpus.valid) return fail(c, FELOG_ERR_STATE, "no corpus");
+    out->round = c->corpus.round;
+    out->max_len = c->corpus.max_len;
+    out->n_walks = c->corpus.n_walks;
+    out->n_tokens = c->corpus.n_tokens;
+    out->n_steps = c->corpus.n_steps;
     return FELOG_OK;
 }
 

@@ -29,6 +29,30 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Per-round values in device memory.  Every kernel of a round reads them
+// there, so a captured round (hipGraph) replays unchanged: the round-start
+// kernel advances the round and its learning rate, and the evaluation
+// writes the stop decision back.
+struct RoundState {
+    uint32_t round;        // 1-based round being run
+    float alpha;           // learning rate of the round
+    uint32_t n_active;     // sentences of the round: walkers started (written by the walker select)
+    uint32_t stop;         // 1 once the plain rule's stop ratio fired
+    uint32_t last_eva;     // unconverged count of the previous evaluation
+    uint32_t eval_from;    // first round that evaluates (min_R)
+    uint32_t budget;       // 1: rounds follow a learned walk budget
+    uint32_t evaluated;    // 1: this round ran the evaluation
+    uint64_t n_tokens;     // corpus tokens of the round
+};
+
+// Whether a captured round evaluates: from round min_R on, while the plain
+// rule has not stopped, and (under a budget) only after a round that walked
+// (src/word2vec.cu:2016-2080; the budget loop stops before training when
+// no walker starts).
+__host__ __device__ inline bool round_evaluates(const RoundState &s) {
+    return s.round >= s.eval_from && !s.stop && !(s.budget && s.n_active == 0);
+}
+
 }  // namespace felog
 
 struct felog_ctx {
@@ -83,7 +107,7 @@ struct felog_ctx {
     // walk buffers
     uint32_t walkers = 0;
     uint32_t *d_active = nullptr;        // walker ids of this round
-    uint8_t *d_walk_ok = nullptr;
+    uint32_t *d_sel = nullptr;           // walkers: inclusive sum of the start flags
     uint32_t *d_slots = nullptr;         // walkers * slot
     uint32_t *d_lens = nullptr;          // walkers
     uint64_t *d_ovf_base = nullptr;      // walkers
@@ -106,12 +130,20 @@ struct felog_ctx {
     void *d_tmp = nullptr;
     size_t tmp_bytes = 0;
 
+    // device round state and the captured round
+    felog::RoundState *d_rs = nullptr;
+    felog::RoundState *h_rs = nullptr;   // pinned mirror
+    hipGraphExec_t round_exec = nullptr;
+    bool round_exec_budget = false;      // the captured round uses the budget
+    bool round_exec_visits = false;      // the captured round adds corpus visits
+
     // learned walk budget: per-vertex walk count and length (0: plain rule)
     bool budget_on = false;
     uint16_t *d_budget_cnt = nullptr, *d_budget_len = nullptr;  // V
     felog_budget_model *d_model = nullptr;
     float *d_feat = nullptr;             // V x FELOG_BUDGET_FEATURES
     uint32_t budget_rounds = 0;          // largest predicted count
+    bool budget_plain_left = false;      // some vertex keeps the plain rule (count 0)
     // corpus visit histogram (cumulative over walk rounds while tracked)
     uint32_t *d_visits = nullptr;        // V
     uint64_t visits_total = 0;
@@ -147,19 +179,41 @@ hipError_t build_csr(felog_ctx *c);
 hipError_t build_alias(felog_ctx *c);
 hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg_host);
 // walk_kernel.hip
-hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active, float *walk_ms);
-hipError_t compact_corpus(felog_ctx *c, uint32_t n_active, float *ms);
+// Round state: eager calls set the round, its learning rate (alpha < 0: the
+// schedule's) and, unless ~0u, the sentence count; captured rounds advance
+// it on the device.
+hipError_t round_set(felog_ctx *c, uint32_t round, float alpha, uint32_t n_sent);
+hipError_t round_next(felog_ctx *c);
+hipError_t round_end(felog_ctx *c);
+hipError_t round_buffers(felog_ctx *c);
+hipError_t walk_enqueue(felog_ctx *c);
+hipError_t compact_enqueue(felog_ctx *c);
+hipError_t run_walks(felog_ctx *c, float *walk_ms);
+hipError_t compact_corpus(felog_ctx *c, float *ms);
+// Schedule of the learning rate: base and per-round decay.
+float alpha_base(const felog_options &o);
+constexpr float kAlphaDecay = 0.1f;
 // sgns_prep.hip (driver) + sgns_tile.hip
 struct SgnsDebug {
     uint32_t max_groups;
     uint32_t *groups, *rows;
     float *logits, *dv, *du;
 };
+// Tile descriptor capacity for corpora of at most n_sent sentences and
+// n_tok tokens (grows the buffer, which drops a captured round).
+hipError_t sgns_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok);
+// The pass over the corpus whose sentence count the round state holds,
+// without a host wait (the tile descriptors must be reserved).
+// mid (when set) is recorded between the prep and tile kernels.
+hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
+                        hipEvent_t mid);
 // ms[0] = whole training pass, ms[1] = prep kernel, ms[2] = tile kernel
 hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent,
-                    uint64_t n_tok, uint32_t round, float alpha, const SgnsDebug *dbg, float *ms);
+                    uint64_t n_tok, const SgnsDebug *dbg, float *ms);
 // eval_kernel.hip
-hipError_t run_eval(felog_ctx *c, uint32_t round, float *ms);
+// gated: evaluate only when the round state says so (captured rounds)
+hipError_t eval_enqueue(felog_ctx *c, bool gated);
+hipError_t run_eval(felog_ctx *c, float *ms);
 // budget_kernels.hip
 constexpr uint32_t kBudgetMaxCount = 255;  // rounds a budget may assign
 hipError_t budget_features(felog_ctx *c, float *d_out);
@@ -167,12 +221,17 @@ hipError_t budget_grad(felog_ctx *c, const float *d_x, const float *d_t, uint64_
                        uint32_t step, uint32_t blocks, const float *d_p, float *d_grad, double *d_loss, float inv_n);
 hipError_t budget_adam(felog_ctx *c, float *d_p, float *d_grad, float *d_m, float *d_v, float lr, uint32_t t);
 hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model);
-hipError_t budget_walk_ok(felog_ctx *c, uint32_t round);
+// start flags (0/1 per walker) into d_ok
+hipError_t budget_walk_ok(felog_ctx *c, uint32_t *d_ok);
 hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok);
+// the round's corpus (token count from the round state)
+hipError_t add_round_visits(felog_ctx *c);
 hipError_t visit_stats(felog_ctx *c, double *d_out);
 hipError_t init_tables(felog_ctx *c);
 
 uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative);
+// Drop the captured round (its buffers are about to change).
+void drop_round_graph(felog_ctx *c);
 // Input rows of an SGNS tile: 48 when a group's windows fit, else 64.
 uint32_t sgns_tile_rows(uint32_t centres, uint32_t window);
 

@@ -23,7 +23,7 @@ constexpr int GLD = OMAX + 4;   // row pitch of the count / G tile (words)
 constexpr int MAXKEEP = 1000;   // MAX_SENTENCE_LENGTH, src/word2vec.cu:42
 constexpr float kMaxExp = 6.f;  // MAX_EXP, src/word2vec.cu:40
 
-// Tile descriptor.  Slot s < n_sent holds the first tile of sentence s
+// Tile descriptor.  Slot s < n_sent (the round's sentences) holds the first tile of sentence s
 // (nc = 0 when the sentence keeps fewer than two tokens); further tiles of
 // long sentences take slots from n_sent on.
 struct TileDesc {
@@ -43,7 +43,8 @@ constexpr int kDescVec = kDescBytes / 16;
 struct Params {
     const uint64_t *__restrict__ off;
     const uint32_t *__restrict__ tok;
-    uint64_t n_sent;
+    // round, learning rate and sentence count (n_active) of the pass
+    const RoundState *__restrict__ round_state;
     void *syn0;
     void *syn1;
     uint32_t dp, d;
@@ -56,8 +57,6 @@ struct Params {
     uint32_t window, negative, bgroup, rmax;
     int compat_break;
     int writeback;  // 1: store staged row + delta; 0: atomic updates
-    float alpha;
-    uint32_t round;
     uint64_t seed;
     unsigned char *tiles;
     uint64_t tiles_cap;

@@ -37,8 +37,10 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     const uint32_t w = a.window, k = a.negative, B = a.bgroup;
     unsigned long long my_words = 0, my_kept = 0;
     const uint64_t nwaves = (uint64_t)gridDim.x * kPrepWaves;
+    const uint64_t n_sent = a.round_state->n_active;
+    const uint32_t round = a.round_state->round;
 
-    for (uint64_t sid = (uint64_t)blockIdx.x * kPrepWaves + wv; sid < a.n_sent; sid += nwaves) {
+    for (uint64_t sid = (uint64_t)blockIdx.x * kPrepWaves + wv; sid < n_sent; sid += nwaves) {
         const uint64_t b0 = a.off[sid];
         const uint32_t n = (uint32_t)(a.off[sid + 1] - b0);
         // ---- filter into stok[0, m) ----
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 else if (!iv) status = 1;
                 else
                     status = (thr == 0xFFFFFFFFu ||
-                              draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, a.round).x < thr) ? 2 : 1;
+                              draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, round).x < thr) ? 2 : 1;
             }
             const uint64_t brk_bal = __ballot(status == 3);
             const uint32_t brk = brk_bal ? chunk + (uint32_t)__ffsll((long long)brk_bal) - 1 : 0xFFFFFFFFu;
@@ -99,13 +101,13 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             // slots after the n_sent first ones
             uint64_t slot = sid;
             if (g0 > 0) {
-                if (lane == 0) slot = a.n_sent + atomicAdd(&a.counters[C_TILES], 1ull);
+                if (lane == 0) slot = n_sent + atomicAdd(&a.counters[C_TILES], 1ull);
                 slot = __shfl(slot, 0);
             }
             // negatives of the tile's groups
             if (lane < nneg) {
                 const uint32_t gq = g0 + lane / k, j = lane % k;
-                const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, a.round);
+                const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, round);
                 const uint32_t bkt = alias_bucket(r.x, a.V);
                 sneg[lane] = r.y < a.uni_thr[bkt] ? bkt : a.uni_alias[bkt];
             }
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                     td->cen_rout[lane] = (valid && lane < nc) ? (uint8_t)my : 0;
                     td->neg_rout[lane] = lane < nneg ? (uint8_t)my_neg : 0;
                     td->bshrink[lane] = lane < nc
-                        ? (uint8_t)(draw(a.seed, c0 + lane, DOM_WINDOW, (uint32_t)sid, a.round).x % w) : 0;
+                        ? (uint8_t)(draw(a.seed, c0 + lane, DOM_WINDOW, (uint32_t)sid, round).x % w) : 0;
                 }
                 if (lane == 0) td->n_out = n_out;
             }
@@ -196,13 +198,29 @@ uint32_t sgns_tile_rows(uint32_t centres, uint32_t window) {
         if (e_ != hipSuccess) return e_;        \
     } while (0)
 
-hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
-                    uint32_t round, float alpha, const SgnsDebug *dbg, float *ms) {
+hipError_t sgns_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok) {
+    using namespace sgns;
+    // every tile holds at least one centre and every sentence ends one tile
+    const uint32_t B = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
+    const uint64_t need = n_tok / B + n_sent + 64;
+    if (need <= c->tiles_cap) return hipSuccess;
+    drop_round_graph(c);
+    dfree(c->d_tiles);
+    c->d_tiles = nullptr;
+    c->tiles_cap = 0;
+    const uint64_t cap = need + need / 4;
+    HIP_TRY(hipMalloc(&c->d_tiles, cap * kDescBytes));
+    c->tiles_cap = cap;
+    return hipSuccess;
+}
+
+hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
+                        hipEvent_t mid) {
     using namespace sgns;
     Params a;
     a.off = d_off;
     a.tok = d_tok;
-    a.n_sent = n_sent;
+    a.round_state = c->d_rs;
     a.syn0 = c->d_syn0;
     a.syn1 = c->d_syn1;
     a.dp = c->dp;
@@ -218,10 +236,7 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
     a.bgroup = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
     a.rmax = sgns_tile_rows(a.bgroup, c->opt.window);
     a.compat_break = c->opt.compat_vocab_break;
-    const int mode = c->opt.update_mode;
-    a.writeback = mode != FELOG_UPDATE_ATOMIC;
-    a.alpha = alpha;
-    a.round = round;
+    a.writeback = c->opt.update_mode != FELOG_UPDATE_ATOMIC;
     a.seed = c->opt.seed;
     a.counters = (unsigned long long *)c->d_counters;
     a.dbg_max = dbg ? dbg->max_groups : 0;
@@ -230,30 +245,21 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
     a.dbg_logits = dbg ? dbg->logits : nullptr;
     a.dbg_dv = dbg ? dbg->dv : nullptr;
     a.dbg_du = dbg ? dbg->du : nullptr;
-    // every tile holds at least one centre and every sentence ends one tile
-    const uint64_t need = n_tok / a.bgroup + n_sent + 64;
-    if (need > c->tiles_cap) {
-        dfree(c->d_tiles);
-        c->d_tiles = nullptr;
-        c->tiles_cap = 0;
-        const uint64_t cap = need + need / 4;
-        HIP_TRY(hipMalloc(&c->d_tiles, cap * kDescBytes));
-        c->tiles_cap = cap;
-    }
     a.tiles = (unsigned char *)c->d_tiles;
     a.tiles_cap = c->tiles_cap;
+    // the sentence count is read on the device: a fixed grid strides over it
+    k_sgns_prep<<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
+    HIP_TRY(hipGetLastError());
+    if (mid) HIP_TRY(hipEventRecord(mid, c->stream));
+    return launch_tile(c, a);
+}
 
+hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
+                    const SgnsDebug *dbg, float *ms) {
+    HIP_TRY(sgns_reserve(c, n_sent, n_tok));
     HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_DEBUG - C_WORDS + 1), c->stream));
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    if (n_sent > 0) {
-        uint64_t pblocks = (n_sent + kPrepWaves - 1) / kPrepWaves;
-        const uint64_t pcap = (uint64_t)c->cu_count * 16;
-        if (pblocks > pcap) pblocks = pcap;
-        k_sgns_prep<<<(unsigned)pblocks, kBlock, 0, c->stream>>>(a);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    if (n_sent > 0) HIP_TRY(launch_tile(c, a));
+    HIP_TRY(sgns_enqueue(c, d_off, d_tok, dbg, c->ev1));
     HIP_TRY(hipEventRecord(c->ev2, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev2));
     HIP_TRY(hipEventElapsedTime(&ms[0], c->ev0, c->ev2));

@@ -218,6 +218,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     T *syn0 = (T *)a.syn0;
     T *syn1 = (T *)a.syn1;
     unsigned long long my_pairs = 0, my_tiles = 0;
+    const uint32_t round = a.round_state->round;
+    const float alpha = a.round_state->alpha;
 
     // gathers: vector v = tid + kBlock*j of a tile covers row v / vpr and
     // 16-byte column v % vpr; (row, column) advance by fixed steps
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     // stochastic-rounding stream of this thread for the launch (bf16 tables)
     uint32_t rs = 0;
     if constexpr (sizeof(T) == 2) {
-        rs = draw(a.seed, blockIdx.x * kBlock + tid, DOM_ROUND_BF16, 0u, a.round).x;
+        rs = draw(a.seed, blockIdx.x * kBlock + tid, DOM_ROUND_BF16, 0u, round).x;
         if (rs == 0) rs = 0x9E3779B9u;
     }
 
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     if constexpr (sizeof(T) == 2)
         for (uint32_t i = tid; i < (RM + OMAX) * ldv * 2 / 16; i += kBlock) ((uint4 *)Vs)[i] = make_uint4(0, 0, 0, 0);
 
-    uint64_t n_tiles = a.n_sent + a.counters[C_TILES];
+    uint64_t n_tiles = (uint64_t)a.round_state->n_active + a.counters[C_TILES];
     if (n_tiles > a.tiles_cap) n_tiles = a.tiles_cap;
     const uint4 *descs = (const uint4 *)a.tiles;
     const uint32_t *desc_words = (const uint32_t *)a.tiles;
@@ -421,10 +423,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
                     cc.w += wv.w << 16;
                 }
                 f4 gv;
-                gv[0] = pair_grad(cc.x, acc[0], a.alpha);
-                gv[1] = pair_grad(cc.y, acc[1], a.alpha);
-                gv[2] = pair_grad(cc.z, acc[2], a.alpha);
-                gv[3] = pair_grad(cc.w, acc[3], a.alpha);
+                gv[0] = pair_grad(cc.x, acc[0], alpha);
+                gv[1] = pair_grad(cc.y, acc[1], alpha);
+                gv[2] = pair_grad(cc.z, acc[2], alpha);
+                gv[3] = pair_grad(cc.w, acc[3], alpha);
                 if constexpr (sizeof(T) == 4) {
                     *(f4 *)cp = gv;
                 } else {

@@ -32,7 +32,7 @@ struct WalkArgs {
     const VertexRec *__restrict__ vrec;
     const EdgeRec *__restrict__ erec;
     const uint32_t *__restrict__ active;
-    uint32_t n_active;
+    const RoundState *rs;  // round and walkers started (device)
     uint32_t V;
     uint32_t *__restrict__ slots;
     uint32_t slot;
@@ -44,7 +44,7 @@ struct WalkArgs {
     const double *__restrict__ T;
     const double *__restrict__ dT;
     const uint16_t *__restrict__ fixed_len;  // per start vertex, 0: stop rule (null: none)
-    uint32_t min_L, max_len, round;
+    uint32_t min_L, max_len;
     uint64_t seed;
 };
 
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
     walk_stats_init(st);
     unsigned long long steps = 0;
     uint32_t maxlen = 0;
+    const uint32_t n_active = a.rs->n_active, round = a.rs->round;
 
     while (true) {
         const uint64_t need = __ballot(!active);
@@ -78,10 +79,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd((unsigned int *)&a.counters[C_WORK], cnt);
             base = __shfl(base, (int)leader);
-            if ((uint64_t)base + cnt >= a.n_active) more = false;
+            if ((uint64_t)base + cnt >= n_active) more = false;
             if (!active) {
                 const uint32_t idx = base + (uint32_t)__popcll(need & lt_mask);
-                if (idx < a.n_active) {
+                if (idx < n_active) {
                     w = a.active[idx];
                     v = w % a.V;
                     lim = a.fixed_len ? a.fixed_len[v] : 0u;
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
             } else if (vr.deg == 2 && a.erec[vr.begin].col == v) {
                 stop = true;
             } else {
-                const u32x4 r = draw(a.seed, s, DOM_WALK | attempt, w, a.round);
+                const u32x4 r = draw(a.seed, s, DOM_WALK | attempt, w, round);
                 const uint32_t j = alias_bucket(r.x, vr.deg);
                 const EdgeRec e = a.erec[vr.begin + j];
                 const uint32_t x = (r.y < e.thresh) ? e.col : e.alias_col;
@@ -168,25 +169,45 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
 }
 
 __global__ void k_walk_ok(const uint8_t *__restrict__ flags, uint32_t walkers, uint32_t V,
-                          uint8_t *__restrict__ ok) {
+                          uint32_t *__restrict__ ok) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < walkers; i += stride)
         ok[i] = flags[i % V] ? 0 : 1;
 }
 
-__global__ void k_gather_lens(const uint32_t *__restrict__ active, uint32_t n,
-                              const uint32_t *__restrict__ lens, uint64_t *__restrict__ out) {
+// After the inclusive sum of the start flags (sel[W-1] = walkers started):
+// walker i starts when the sum steps at i, and goes to position sel[i] - 1
+// of the active list, which keeps walker-id order.
+__global__ void k_select(const uint32_t *__restrict__ sel, uint32_t W, uint32_t *__restrict__ active,
+                         RoundState *__restrict__ rs) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        out[i] = lens[active[i]];
+    if (blockIdx.x == 0 && threadIdx.x == 0) rs->n_active = W ? sel[W - 1] : 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < W; i += stride) {
+        const uint32_t p = sel[i], q = i ? sel[i - 1] : 0;
+        if (p != q) active[q] = i;
+    }
+}
+
+// Lengths of this round's walks into off[1 .. W] (0 past the walkers
+// started) and off[0] = 0, so the offsets scan runs over a fixed W entries.
+__global__ void k_gather_lens(const uint32_t *__restrict__ active, const RoundState *__restrict__ rs, uint32_t W,
+                              const uint32_t *__restrict__ lens, uint64_t *__restrict__ off) {
+    const uint32_t n = rs->n_active;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[0] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < W; i += stride)
+        off[1 + i] = i < n ? lens[active[i]] : 0;
 }
 
 // One wavefront per walk: lanes copy consecutive vertices of the path.
-__global__ void k_compact(const uint32_t *__restrict__ active, uint32_t n,
-                          const uint64_t *__restrict__ off, const uint32_t *__restrict__ slots,
+// Thread 0 also records the round's token count.
+__global__ void k_compact(const uint32_t *__restrict__ active, RoundState *__restrict__ rs,
+                          uint64_t *__restrict__ off, const uint32_t *__restrict__ slots,
                           uint32_t slot, const uint64_t *__restrict__ ovf_base,
                           const uint32_t *__restrict__ ovf, uint32_t *__restrict__ tok,
                           uint32_t *__restrict__ sent_id) {
+    const uint32_t n = rs->n_active;
+    if (blockIdx.x == 0 && threadIdx.x == 0) rs->n_tokens = off[n];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -198,6 +219,48 @@ __global__ void k_compact(const uint32_t *__restrict__ active, uint32_t n,
         for (uint64_t k = lane; k < len; k += 64)
             tok[b + k] = k < slot ? slots[(uint64_t)w * slot + k] : ovf[ob + (k - slot)];
     }
+}
+
+// Learning rate of round r: ExponentialDecayLR(base, decay) with one
+// get_lr() call per round (include/lr_scheduler.hpp:23-31,
+// src/word2vec.cu:1958,2001).  Eager and captured rounds both take it from
+// here, so they train with the same bits.
+__device__ __forceinline__ float scheduled_alpha(float base, float decay, uint32_t r) {
+    return base * expf(-decay * (float)(r - 1));
+}
+
+// Eager calls: the round, its learning rate (alpha < 0: the schedule's) and,
+// unless n_sent is ~0u, the sentence count of the corpus to train.
+__global__ void k_round_set(RoundState *rs, uint32_t round, float alpha, float base, float decay, uint32_t n_sent) {
+    rs->round = round;
+    rs->alpha = alpha < 0.f ? scheduled_alpha(base, decay, round) : alpha;
+    if (n_sent != ~0u) rs->n_active = n_sent;
+}
+
+// Captured rounds: advance to the next round and its learning rate, and
+// clear the per-round counters (a kernel, not a memset node: such nodes are
+// avoided in captured graphs).
+__global__ void k_round_next(RoundState *rs, float base, float decay, unsigned long long *counters, int n) {
+    if (threadIdx.x == 0) {
+        const uint32_t r = rs->round + 1;
+        rs->round = r;
+        rs->alpha = scheduled_alpha(base, decay, r);
+        rs->evaluated = 0;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) counters[i] = 0;
+}
+
+// Captured rounds: the plain rule's stop decision after the evaluation
+// (src/word2vec.cu:2073-2078: stop once the unconverged count shrinks by
+// less than 30% in a round), in the host's float arithmetic.
+__global__ void k_round_end(RoundState *rs, const unsigned long long *counters) {
+    RoundState &s = *rs;
+    if (!round_evaluates(s)) return;
+    const uint32_t eva = (uint32_t)counters[C_EVALNUM];
+    const float ratio = s.last_eva ? (float)eva / (float)s.last_eva : 0.f;
+    if (s.last_eva != 0 && ratio > 0.7f) s.stop = 1;
+    s.last_eva = eva;
+    s.evaluated = 1;
 }
 
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap) {
@@ -215,45 +278,61 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap) {
         if (e_ != hipSuccess) return e_;        \
     } while (0)
 
-// Builds the active list (walkers whose start vertex is unconverged, in
-// walker-id order, include/walk.hpp:561-576) and runs the persistent walk.
-hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active_unused, float *walk_ms) {
-    (void)n_active_unused;
+// Scratch of the per-round library primitives (walker select, offsets
+// scan over W entries) and the fixed-capacity corpus buffers, sized once so
+// a round allocates nothing and can be captured.
+hipError_t round_buffers(felog_ctx *c) {
     const uint32_t W = c->walkers;
-    if (c->budget_on) {
-        HIP_TRY(budget_walk_ok(c, round));
-    } else {
-        k_walk_ok<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_flags, W, c->V, c->d_walk_ok);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipMemsetAsync(c->d_counters, 0, kNumCounters * 8, c->stream));
-    size_t tmp = 0;
-    hipcub::CountingInputIterator<uint32_t> ids(0);
-    unsigned long long *d_nsel = (unsigned long long *)(c->d_counters + C_EVALCONV + 2);
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tmp, ids, c->d_walk_ok, c->d_active, d_nsel, W,
-                                          c->stream));
+    size_t t1 = 0, t2 = 0;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, t1, c->d_active, c->d_sel, W, c->stream));
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, t2, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint64_t)W,
+                                             c->stream));
+    const size_t tmp = t1 > t2 ? t1 : t2;
     if (tmp > c->tmp_bytes) {
         dfree(c->d_tmp);
         HIP_TRY(hipMalloc(&c->d_tmp, tmp));
         c->tmp_bytes = tmp;
     }
-    HIP_TRY(hipcub::DeviceSelect::Flagged(c->d_tmp, tmp, ids, c->d_walk_ok, c->d_active, d_nsel, W,
-                                          c->stream));
-    HIP_TRY(hipMemcpyAsync(c->h_counters, c->d_counters, kNumCounters * 8, hipMemcpyDeviceToHost,
-                           c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const uint32_t n_active = (uint32_t)c->h_counters[C_EVALCONV + 2];
-    c->corpus.n_walks = n_active;
-    if (n_active == 0) {
-        *walk_ms = 0.f;
-        return hipSuccess;
+    if ((uint64_t)W + 1 > c->off_cap) {
+        dfree(c->d_off);
+        dfree(c->d_sent_id);
+        HIP_TRY(hipMalloc(&c->d_off, ((uint64_t)W + 1) * 8));
+        HIP_TRY(hipMalloc(&c->d_sent_id, ((uint64_t)W + 1) * 4));
+        c->off_cap = (uint64_t)W + 1;
     }
+    // every token of a round sits in a walker's slot or the overflow pool
+    const uint64_t tok = (uint64_t)W * c->slot + c->ovf_cap;
+    if (tok > c->tok_cap) {
+        dfree(c->d_tok);
+        HIP_TRY(hipMalloc(&c->d_tok, tok * 4));
+        c->tok_cap = tok;
+    }
+    return hipSuccess;
+}
 
+// Walker select (walkers whose start vertex is unconverged or, under a
+// learned budget, whose count covers the round; walker-id order,
+// include/walk.hpp:561-576) and the persistent walk, without a host wait.
+// The select is a scan of the start flags (held in the active list's
+// buffer) and a scatter: kernels only, so it can be captured (a library
+// select clears its counter with a memset).
+hipError_t walk_enqueue(felog_ctx *c) {
+    const uint32_t W = c->walkers;
+    if (c->budget_on) {
+        HIP_TRY(budget_walk_ok(c, c->d_active));
+    } else {
+        k_walk_ok<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_flags, W, c->V, c->d_active);
+        HIP_TRY(hipGetLastError());
+    }
+    size_t tmp = c->tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(c->d_tmp, tmp, c->d_active, c->d_sel, W, c->stream));
+    k_select<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_sel, W, c->d_active, c->d_rs);
+    HIP_TRY(hipGetLastError());
     WalkArgs a;
     a.vrec = c->d_vrec;
     a.erec = c->d_erec;
     a.active = c->d_active;
-    a.n_active = n_active;
+    a.rs = c->d_rs;
     a.V = c->V;
     a.slots = c->d_slots;
     a.slot = c->slot;
@@ -267,61 +346,67 @@ hipError_t run_walks(felog_ctx *c, uint32_t round, uint32_t n_active_unused, flo
     a.fixed_len = c->budget_on ? c->d_budget_len : nullptr;
     a.min_L = c->opt.min_L;
     a.max_len = c->opt.max_walk_len;
-    a.round = round;
     a.seed = c->opt.seed;
-
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_walk, kWalkBlock, 0));
     if (per_cu < 1) per_cu = 1;
     uint64_t blocks = (uint64_t)per_cu * c->cu_count;
-    const uint64_t need_blocks = ((uint64_t)n_active + kWalkBlock - 1) / kWalkBlock;
+    const uint64_t need_blocks = ((uint64_t)W + kWalkBlock - 1) / kWalkBlock;
     if (blocks > need_blocks) blocks = need_blocks;
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
     k_walk<<<(unsigned)blocks, kWalkBlock, 0, c->stream>>>(a);
+    return hipGetLastError();
+}
+
+// Offsets (scan over W lengths) and the flat corpus, without a host wait.
+hipError_t compact_enqueue(felog_ctx *c) {
+    const uint32_t W = c->walkers;
+    k_gather_lens<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_active, c->d_rs, W, c->d_lens, c->d_off);
     HIP_TRY(hipGetLastError());
+    size_t tmp = c->tmp_bytes;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(c->d_tmp, tmp, c->d_off + 1, c->d_off + 1, (uint64_t)W, c->stream));
+    k_compact<<<grid_for((uint64_t)W * 64, 256, 1u << 16), 256, 0, c->stream>>>(
+        c->d_active, c->d_rs, c->d_off, c->d_slots, c->slot, c->d_ovf_base, c->d_ovf, c->d_tok, c->d_sent_id);
+    return hipGetLastError();
+}
+
+hipError_t round_set(felog_ctx *c, uint32_t round, float alpha, uint32_t n_sent) {
+    k_round_set<<<1, 1, 0, c->stream>>>(c->d_rs, round, alpha, alpha_base(c->opt), kAlphaDecay, n_sent);
+    return hipGetLastError();
+}
+
+hipError_t round_next(felog_ctx *c) {
+    k_round_next<<<1, 64, 0, c->stream>>>(c->d_rs, alpha_base(c->opt), kAlphaDecay,
+                                          (unsigned long long *)c->d_counters, kNumCounters);
+    return hipGetLastError();
+}
+
+hipError_t round_end(felog_ctx *c) {
+    k_round_end<<<1, 1, 0, c->stream>>>(c->d_rs, (const unsigned long long *)c->d_counters);
+    return hipGetLastError();
+}
+
+// Eager round: walk, then read the walker count and the counters.
+hipError_t run_walks(felog_ctx *c, float *walk_ms) {
+    HIP_TRY(hipMemsetAsync(c->d_counters, 0, kNumCounters * 8, c->stream));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(walk_enqueue(c));
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
     HIP_TRY(hipEventElapsedTime(walk_ms, c->ev0, c->ev1));
     HIP_TRY(hipMemcpy(c->h_counters, c->d_counters, kNumCounters * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(c->h_rs, c->d_rs, sizeof(RoundState), hipMemcpyDeviceToHost));
+    c->corpus.n_walks = c->h_rs->n_active;
     return hipSuccess;
 }
 
-hipError_t compact_corpus(felog_ctx *c, uint32_t n, float *ms) {
-    if (n + 1 > c->off_cap) {
-        dfree(c->d_off);
-        dfree(c->d_sent_id);
-        HIP_TRY(hipMalloc(&c->d_off, ((uint64_t)n + 1) * 8));
-        HIP_TRY(hipMalloc(&c->d_sent_id, ((uint64_t)n + 1) * 4));
-        c->off_cap = (uint64_t)n + 1;
-    }
+hipError_t compact_corpus(felog_ctx *c, float *ms) {
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    k_gather_lens<<<grid_for(n, 256, 65535), 256, 0, c->stream>>>(c->d_active, n, c->d_lens, c->d_off + 1);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemsetAsync(c->d_off, 0, 8, c->stream));
-    size_t tmp = 0;
-    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, c->d_off + 1, c->d_off + 1, (uint64_t)n, c->stream));
-    if (tmp > c->tmp_bytes) {
-        dfree(c->d_tmp);
-        HIP_TRY(hipMalloc(&c->d_tmp, tmp));
-        c->tmp_bytes = tmp;
-    }
-    HIP_TRY(hipcub::DeviceScan::InclusiveSum(c->d_tmp, tmp, c->d_off + 1, c->d_off + 1, (uint64_t)n, c->stream));
-    uint64_t total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, c->d_off + n, 8, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (total > c->tok_cap) {
-        dfree(c->d_tok);
-        const uint64_t cap = total + total / 8 + 1024;
-        HIP_TRY(hipMalloc(&c->d_tok, cap * 4));
-        c->tok_cap = cap;
-    }
-    k_compact<<<grid_for((uint64_t)n * 64, 256, 1u << 16), 256, 0, c->stream>>>(
-        c->d_active, n, c->d_off, c->d_slots, c->slot, c->d_ovf_base, c->d_ovf, c->d_tok, c->d_sent_id);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(compact_enqueue(c));
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
     HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));
-    c->corpus.n_tokens = total;
+    HIP_TRY(hipMemcpy(c->h_rs, c->d_rs, sizeof(RoundState), hipMemcpyDeviceToHost));
+    c->corpus.n_tokens = c->h_rs->n_tokens;
     return hipSuccess;
 }
 

@@ -61,7 +61,7 @@ class Stats(C.Structure):
         ("build_ms", C.c_double), ("walk_kernel_ms", C.c_double), ("compact_kernel_ms", C.c_double),
         ("train_kernel_ms", C.c_double), ("eval_kernel_ms", C.c_double),
         ("walk_ms_total", C.c_double), ("train_ms_total", C.c_double), ("eval_ms_total", C.c_double),
-        ("sgns_prep_ms", C.c_double), ("sgns_tile_ms", C.c_double),
+        ("sgns_prep_ms", C.c_double), ("sgns_tile_ms", C.c_double), ("graph_ms_total", C.c_double),
     ]
 
 
@@ -127,6 +127,8 @@ def load_library(path=LIB_PATH):
         "felog_budget_set": (C.c_int32, [P, C.POINTER(BudgetModel)]),
         "felog_budget_copy_out": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32)]),
         "felog_run_budget": (C.c_int32, [P]),
+        "felog_run_graph": (C.c_int32, [P, C.c_uint32]),
+        "felog_corpus_get": (C.c_int32, [P, C.POINTER(Corpus)]),
         "felog_visits_reset": (C.c_int32, [P]),
         "felog_visit_stats": (C.c_int32, [P, P]),
     }
@@ -211,12 +213,13 @@ class Felog:
     def walk_round(self, rnd):
         c = Corpus()
         self._check(self.lib.felog_walk_round(self.ctx, rnd, C.byref(c)), "felog_walk_round")
-        self._corpus = c
         return {"round": c.round, "max_len": c.max_len, "n_walks": c.n_walks,
                 "n_tokens": c.n_tokens, "n_steps": c.n_steps}
 
     def train_round(self):
-        self._check(self.lib.felog_train_round(self.ctx, C.byref(self._corpus)), "felog_train_round")
+        c = Corpus()
+        self._check(self.lib.felog_corpus_get(self.ctx, C.byref(c)), "felog_corpus_get")
+        self._check(self.lib.felog_train_round(self.ctx, C.byref(c)), "felog_train_round")
 
     def train_corpus(self, offsets, tokens, rnd):
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
@@ -273,6 +276,11 @@ class Felog:
     def run_budget(self):
         self._check(self.lib.felog_run_budget(self.ctx), "felog_run_budget")
 
+    def run_graph(self, max_rounds=0):
+        """run() / run_budget() with each round replayed from a hipGraph;
+        at most max_rounds more rounds (0: until the stop rule)."""
+        self._check(self.lib.felog_run_graph(self.ctx, max_rounds), "felog_run_graph")
+
     def visits_reset(self):
         self._check(self.lib.felog_visits_reset(self.ctx), "felog_visits_reset")
 
@@ -283,7 +291,8 @@ class Felog:
 
     # -- read-back -------------------------------------------------------
     def corpus(self):
-        c = self._corpus
+        c = Corpus()
+        self._check(self.lib.felog_corpus_get(self.ctx, C.byref(c)), "felog_corpus_get")
         off = np.empty(c.n_walks + 1, np.uint64)
         tok = np.empty(max(c.n_tokens, 1), np.uint32)
         wid = np.empty(max(c.n_walks, 1), np.uint32)
