@@ -2,18 +2,21 @@
 """Benchmark of the FeLoG walk -> SGNS hot path on one MI355X.
 
 One step = one round of the path: a walk of every walker over the resident
-graph (persistent walk kernel + corpus compaction), then skip-gram training
-over that round's corpus (SGNS kernels).  The graph, alias tables and both
-embedding tables are built in HBM before the timed region; nothing crosses
-PCIe inside it.
+graph (walker select, persistent walk kernel, corpus compaction), then
+skip-gram training over that round's corpus (SGNS kernels).  The graph, alias
+tables and both embedding tables are built in HBM before the timed region;
+nothing crosses PCIe inside it.  The timed steps run as captured rounds
+(felog_run_graph: one hipGraph replay per round, one host wait per round);
+the per-kernel times come from as many eager rounds before them, with HIP
+events around each launch.
 
 Workload (default): Chung-Lu power law (exponent 2.2), 2.24M vertices,
 ~35M undirected edges, weights U(0,1], info-centric walks (min_L=20), plain
 FP32 HogBatch SGNS with d=128, window 10, negative 5, batch 21.
 
 Prints one JSON line.  `value` is corpus tokens walked and trained per
-second of the whole round (plain FP32 form); walk steps/s and SGNS words/s
-are reported beside it.  The roofline is that of the kernel that takes the
+second of the whole round (plain FP32 form, captured rounds); the eager
+rounds' rate, walk steps/s and SGNS words/s are reported beside it.  The roofline is that of the kernel that takes the
 most time, from HIP events recorded on the library's stream around each
 launch.  The `e2e` object holds the end-to-end comparison the metric names:
 on the same graph with 10% of its edges held out, the plain FP32 form run
@@ -110,8 +113,9 @@ def train_budget_model(args):
 
 
 def end_to_end(src, dst, w, n, args, model):
-    """Plain FP32 run (felog_run) against BF16 + learned budget
-    (felog_run_budget) on the graph with 10% of its edges held out."""
+    """Plain FP32 run (felog_run's rule) against BF16 + learned budget
+    (felog_run_budget's rule) on the graph with 10% of its edges held out;
+    both as captured rounds (felog_run_graph)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
@@ -129,18 +133,14 @@ def end_to_end(src, dst, w, n, args, model):
             extra = {"budgeted_vertices": int((cnt > 0).sum()), "plain_rule_fallback": int((cnt == 0).sum()),
                      "max_count": int(rounds)}
         t0 = time.perf_counter()
-        if budgeted:
-            f.run_budget()
-        else:
-            f.run()
+        f.run_graph()  # felog_run_budget / felog_run as captured rounds
         wall = time.perf_counter() - t0
         st = f.stats()
         vs = f.visit_stats()
         emb, _ = f.embeddings()
         out[form] = dict(wall_ms=wall * 1e3, rounds=st["rounds_done"], tokens=vs["visits"],
                          tokens_per_s=vs["visits"] / wall, auc=_auc(emb, pos, neg, orc), visit_kl=vs["kl"],
-                         visit_entropy=vs["entropy"], walk_ms=st["walk_ms_total"], train_ms=st["train_ms_total"],
-                         eval_ms=st["eval_ms_total"], **extra)
+                         visit_entropy=vs["entropy"], rounds_gpu_ms=st["graph_ms_total"], **extra)
         f.close()
     p, b = out["plain_fp32"], out["bf16_surrogate"]
     out["throughput_ratio"] = p["wall_ms"] / b["wall_ms"]
@@ -208,8 +208,9 @@ def main():
           file=sys.stderr, flush=True)
     prec = felog.BF16 if args.precision == "bf16" else felog.FP32
     esize = 2 if prec == felog.BF16 else 4
+    # every round walks every vertex: no convergence evaluation in the steps
     f = felog.Felog(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20,
-                    min_R=2, precision=prec, seed=args.seed)
+                    min_R=1 << 30, precision=prec, seed=args.seed)
     f.set_edges(src, dst, w, n, make_undirected=True)
     f.build()
     st = f.stats()
@@ -238,7 +239,17 @@ def main():
         train_ms += s["train_kernel_ms"]
         prep_ms += s["sgns_prep_ms"]
         tile_ms += s["sgns_tile_ms"]
+    eager_elapsed = time.perf_counter() - t0
+    eager_tok = tok
+
+    # timed region: K captured rounds (inputs resident; run_graph returns
+    # after its last round's host wait)
+    f.run_graph(1)  # capture + one warm replay
+    tok0 = f.stats()["walk_tokens_total"]
+    t0 = time.perf_counter()
+    f.run_graph(args.steps)
     elapsed = time.perf_counter() - t0
+    tok = f.stats()["walk_tokens_total"] - tok0
 
     K = args.steps
     walk_gbs = steps * 24 / (walk_ms / 1e3) / 1e9
@@ -272,6 +283,8 @@ def main():
         "data": "synthetic (seeded generator; random-init tables)",
         "config": {"workload": f"{args.workload} {args.precision}", "vertices": n, "undirected_edges": int(len(src)),
                    "dim": args.dim, "window": 10, "negative": 5, "batch": 21, "min_L": 20, "sample": 1e-3},
+        "eager_tokens_per_s": eager_tok / eager_elapsed,
+        "eager_ms_per_step": eager_elapsed / K * 1e3,
         "walk_steps_per_s": steps / (walk_ms / 1e3),
         "sgns_words_per_s": words / (train_ms / 1e3),
         "walk_kernel_ms": walk_ms / K,
@@ -280,6 +293,7 @@ def main():
         "sgns_prep_kernel_ms": prep_ms / K,
         "sgns_tile_kernel_ms": tile_ms / K,
         "tokens_per_step": tok / K,
+        "eager_tokens_per_step": eager_tok / K,
         "pairs_per_step": pairs / K,
         "walk_roofline": {"bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": walk_gbs / HBM_PEAK_GBS, "bytes_per_step": 24},

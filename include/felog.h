@@ -137,6 +137,7 @@ typedef struct felog_stats {
     double sgns_tile_ms;             /* last round: SGNS tile kernel        */
     double graph_ms_total;           /* captured rounds (felog_run_graph),
                                         HIP events around each replay       */
+    uint64_t walk_tokens_total;      /* corpus tokens of all walk rounds     */
 } felog_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */

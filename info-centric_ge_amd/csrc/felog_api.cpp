@@ -311,6 +311,7 @@ felog_status felog_walk_round(felog_ctx *c, uint32_t round, felog_corpus *out) {
     c->corpus.from_walk = true;
     c->st.walk_steps = steps;
     c->st.walk_tokens = c->corpus.n_tokens;
+    c->st.walk_tokens_total += c->corpus.n_tokens;
     c->st.walk_kernel_ms = wms;
     c->st.compact_kernel_ms = cms;
     c->st.walk_ms_total += wms + cms;
@@ -722,6 +723,7 @@ felog_status felog_run_graph(felog_ctx *c, uint32_t max_rounds) {
         c->corpus.from_walk = true;
         c->st.walk_steps = k[C_STEPS];
         c->st.walk_tokens = h.n_tokens;
+        c->st.walk_tokens_total += h.n_tokens;
         c->st.overflow_used = k[C_OVF_NEXT];
         c->st.train_words = k[C_WORDS];
         c->st.train_kept = k[C_KEPT];

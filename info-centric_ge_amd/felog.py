@@ -62,6 +62,7 @@ class Stats(C.Structure):
         ("train_kernel_ms", C.c_double), ("eval_kernel_ms", C.c_double),
         ("walk_ms_total", C.c_double), ("train_ms_total", C.c_double), ("eval_ms_total", C.c_double),
         ("sgns_prep_ms", C.c_double), ("sgns_tile_ms", C.c_double), ("graph_ms_total", C.c_double),
+        ("walk_tokens_total", C.c_uint64),
     ]
 
 

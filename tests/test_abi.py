@@ -45,7 +45,7 @@ def test_options_struct_layout_matches_header():
     assert ctypes.sizeof(felog.Options) == 104
     assert felog.Options.seed.offset == 72 and felog.Options.device.offset == 92
     assert felog.Options.update_mode.offset == 96
-    assert ctypes.sizeof(felog.Stats) == 8 * 3 + 4 * 2 + 8 * 7 + 8 * 11
+    assert ctypes.sizeof(felog.Stats) == 8 * 3 + 4 * 2 + 8 * 7 + 8 * 11 + 8
 
 
 @pytest.mark.parametrize("bad", [dict(size=0), dict(size=600), dict(window=0), dict(window=40),
