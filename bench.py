@@ -125,6 +125,10 @@ def end_to_end(src, dst, w, n, args, model):
                         precision=prec, seed=args.seed)
         f.set_edges(ts, td, tw, n, make_undirected=True)
         f.build()
+        # tile shape for this precision, timed on a first-round corpus
+        f.walk_round(1)
+        f.tune_sgns(sample=0, reps=1)
+        shape = f.tile_shape()
         f.visits_reset()
         extra = {}
         if budgeted:
@@ -140,7 +144,8 @@ def end_to_end(src, dst, w, n, args, model):
         emb, _ = f.embeddings()
         out[form] = dict(wall_ms=wall * 1e3, rounds=st["rounds_done"], tokens=vs["visits"],
                          tokens_per_s=vs["visits"] / wall, auc=_auc(emb, pos, neg, orc), visit_kl=vs["kl"],
-                         visit_entropy=vs["entropy"], rounds_gpu_ms=st["graph_ms_total"], **extra)
+                         visit_entropy=vs["entropy"], rounds_gpu_ms=st["graph_ms_total"],
+                         tile_shape={k: shape[k] for k in ("rows", "waves_per_simd", "groups_per_cu")}, **extra)
         f.close()
     p, b = out["plain_fp32"], out["bf16_surrogate"]
     out["throughput_ratio"] = p["wall_ms"] / b["wall_ms"]
@@ -223,6 +228,15 @@ def main():
 
     for r in range(1, args.warmup + 1):
         one_round(r)
+    # the autotuner times every tile shape on the last warm-up corpus (tables
+    # restored after each pass) and installs the fastest
+    default_shape = f.tile_shape()
+    t0 = time.perf_counter()
+    tuned = f.tune_sgns(sample=0, reps=2)
+    tune_s = time.perf_counter() - t0
+    chosen = f.tile_shape()
+    print(f"[bench] tile autotune {len(tuned)} shapes in {tune_s:.1f}s: chosen {chosen}", file=sys.stderr,
+          flush=True)
     tok = steps = words = kept = pairs = 0
     walk_ms = comp_ms = train_ms = prep_ms = tile_ms = 0.0
     t0 = time.perf_counter()
@@ -302,6 +316,13 @@ def main():
                           "bytes_per_kept_word": sgns_bytes_per_word(args.dim, 5, 21, esize),
                           "flop_per_pair": 6 * 6 * args.dim},
         "roofline": roof,
+        "tile_autotune": {
+            "chosen": {k: chosen[k] for k in ("rows", "waves_per_simd", "groups_per_cu")},
+            "default": {k: default_shape[k] for k in ("rows", "waves_per_simd", "groups_per_cu")},
+            "seconds": tune_s,
+            "shapes": [[t["rows"], t["waves_per_simd"], t["groups_per_cu"], round(t["tile_ms"], 3)]
+                       for t in tuned],
+            "fields": "rows, waves_per_simd, groups_per_cu, tile kernel ms on the warm-up corpus"},
         "cpu_baseline": None,
     }
     f.close()

@@ -191,6 +191,28 @@ felog_status felog_run(felog_ctx *ctx);
  * max_rounds); a later call continues from the rounds done. */
 felog_status felog_run_graph(felog_ctx *ctx, uint32_t max_rounds);
 
+/* ---- SGNS tile shape and its autotuner ---------------------------------- */
+/* Shape of the HogBatch tile kernel (no reference counterpart; the
+ * reference's HogBatch sizes its MKL GEMMs per sentence pair,
+ * include/dsgl.hpp:1099-1159).  Zero fields take the defaults. */
+typedef struct felog_tile_shape {
+    uint32_t rows;            /* input positions (windows) of a tile: 48 or 64;
+                                 48 needs batch centres + 2 window <= 48      */
+    uint32_t waves_per_simd;  /* register budget: FP32 2 or 3, BF16 3 or 4  */
+    uint32_t groups_per_cu;   /* persistent workgroups per CU (at most what
+                                 LDS and registers allow)                   */
+    float tile_ms;            /* felog_tune_sgns: tile-kernel time          */
+} felog_tile_shape;
+felog_status felog_tile_shape_set(felog_ctx *ctx, const felog_tile_shape *shape);
+felog_status felog_tile_shape_get(felog_ctx *ctx, felog_tile_shape *shape);
+/* Times every tile shape on the first sample_sentences sentences of the
+ * current walk corpus (reps timed passes each, after one untimed), with the
+ * tables restored after every pass, and installs the fastest.  results
+ * (up to max_results) receive every shape tried with its mean time. */
+felog_status felog_tune_sgns(felog_ctx *ctx, uint64_t sample_sentences,
+                             uint32_t reps, felog_tile_shape *results,
+                             uint32_t max_results, uint32_t *n_results);
+
 /* ---- read-back (caller-allocated host buffers) ------------------------- */
 /* Descriptor of the context's current corpus: the last walk round's, also
  * after felog_run and felog_run_graph. */

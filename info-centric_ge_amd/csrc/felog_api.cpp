@@ -781,6 +781,122 @@ felog_status felog_visit_stats(felog_ctx *c, double out[4]) {
     return FELOG_OK;
 }
 
+// ---- SGNS tile shape and autotuner -----------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+uint32_t group_centres(const felog_ctx *c) {
+    return sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
+}
+
+felog_status check_shape(felog_ctx *c, const felog_tile_shape &s) {
+    if (s.rows != 0 && s.rows != 48 && s.rows != 64) return fail(c, FELOG_ERR_ARG, "tile rows must be 48 or 64");
+    if (s.rows == 48 && group_centres(c) + 2 * c->opt.window > 48)
+        return fail(c, FELOG_ERR_ARG, "48-row tiles need batch centres + 2 window <= 48");
+    const uint32_t lo = tile_wpe_low(c);
+    if (s.waves_per_simd != 0 && s.waves_per_simd != lo && s.waves_per_simd != lo + 1)
+        return fail(c, FELOG_ERR_ARG,
+                    "waves_per_simd must be " + std::to_string(lo) + " or " + std::to_string(lo + 1));
+    if (s.groups_per_cu > 64) return fail(c, FELOG_ERR_ARG, "groups_per_cu must be <= 64");
+    return FELOG_OK;
+}
+
+void apply_shape(felog_ctx *c, uint32_t rows, uint32_t wpe, uint32_t gpc) {
+    if (rows != c->tile_rows || wpe != c->tile_wpe || gpc != c->tile_gpc) drop_round_graph(c);
+    c->tile_rows = rows;
+    c->tile_wpe = wpe;
+    c->tile_gpc = gpc;
+}
+
+}  // namespace
+
+extern "C" {
+
+felog_status felog_tile_shape_set(felog_ctx *c, const felog_tile_shape *shape) {
+    if (!c) return FELOG_ERR_ARG;
+    const felog_tile_shape zero{};
+    const felog_tile_shape &s = shape ? *shape : zero;
+    felog_status st = check_shape(c, s);
+    if (st) return st;
+    apply_shape(c, s.rows, s.waves_per_simd, s.groups_per_cu);
+    return FELOG_OK;
+}
+
+felog_status felog_tile_shape_get(felog_ctx *c, felog_tile_shape *out) {
+    if (!c || !out) return FELOG_ERR_ARG;
+    TRY(c, hipSetDevice(c->device));
+    out->rows = c->tile_rows ? c->tile_rows : sgns_tile_rows(group_centres(c), c->opt.window);
+    out->waves_per_simd = c->tile_wpe ? c->tile_wpe : tile_wpe_low(c) + 1;
+    int per_cu = 0;
+    TRY(c, tile_occupancy(c, out->rows, out->waves_per_simd, &per_cu));
+    out->groups_per_cu = c->tile_gpc && (int)c->tile_gpc < per_cu ? c->tile_gpc : (uint32_t)per_cu;
+    out->tile_ms = 0.f;
+    return FELOG_OK;
+}
+
+felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t reps, felog_tile_shape *results,
+                             uint32_t max_results, uint32_t *n_results) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!c->corpus.valid || !c->corpus.from_walk)
+        return fail(c, FELOG_ERR_STATE, "no walk corpus: call felog_walk_round first");
+    if (reps == 0) reps = 1;
+    uint64_t n = c->corpus.n_walks;
+    if (sample_sentences && sample_sentences < n) n = sample_sentences;
+    if (n == 0) return fail(c, FELOG_ERR_STATE, "the walk corpus is empty");
+    uint64_t nt = 0;
+    TRY(c, hipMemcpy(&nt, c->d_off + n, 8, hipMemcpyDeviceToHost));
+    // every shape: rows x register budget x workgroups per CU up to what fits
+    std::vector<felog_tile_shape> cand;
+    const uint32_t lo = tile_wpe_low(c);
+    for (uint32_t rows : {48u, 64u}) {
+        if (rows == 48 && group_centres(c) + 2 * c->opt.window > 48) continue;
+        for (uint32_t wpe : {lo, lo + 1}) {
+            int per_cu = 0;
+            TRY(c, tile_occupancy(c, rows, wpe, &per_cu));
+            for (int g = 1; g <= per_cu; ++g) cand.push_back(felog_tile_shape{rows, wpe, (uint32_t)g, 0.f});
+        }
+    }
+    // both tables (one allocation) are restored before every pass
+    const size_t tbytes = 2 * (size_t)c->V * c->dp * (c->opt.precision == FELOG_BF16 ? 2 : 4);
+    void *snap = nullptr;
+    TRY(c, hipMalloc(&snap, tbytes));
+    const uint32_t keep_rows = c->tile_rows, keep_wpe = c->tile_wpe, keep_gpc = c->tile_gpc;
+    hipError_t e = hipMemcpyAsync(snap, c->d_syn0, tbytes, hipMemcpyDeviceToDevice, c->stream);
+    for (size_t i = 0; i < cand.size() && e == hipSuccess; ++i) {
+        apply_shape(c, cand[i].rows, cand[i].waves_per_simd, cand[i].groups_per_cu);
+        double sum = 0.0;
+        for (uint32_t r = 0; r <= reps && e == hipSuccess; ++r) {
+            e = hipMemcpyAsync(c->d_syn0, snap, tbytes, hipMemcpyDeviceToDevice, c->stream);
+            if (e == hipSuccess) e = round_set(c, c->corpus.round, -1.f, (uint32_t)n);
+            float ms[3] = {0.f, 0.f, 0.f};
+            if (e == hipSuccess) e = run_sgns(c, c->d_off, c->d_tok, n, nt, nullptr, ms);
+            if (r > 0) sum += ms[2];
+        }
+        cand[i].tile_ms = (float)(sum / reps);
+    }
+    const hipError_t e2 = hipMemcpyAsync(c->d_syn0, snap, tbytes, hipMemcpyDeviceToDevice, c->stream);
+    const hipError_t e3 = hipStreamSynchronize(c->stream);
+    dfree(snap);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = e3;
+    if (e != hipSuccess) {
+        apply_shape(c, keep_rows, keep_wpe, keep_gpc);
+        return hip_fail(c, e, "felog_tune_sgns");
+    }
+    size_t best = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (cand[i].tile_ms < cand[best].tile_ms) best = i;
+    apply_shape(c, cand[best].rows, cand[best].waves_per_simd, cand[best].groups_per_cu);
+    const uint32_t m = (uint32_t)(cand.size() < max_results ? cand.size() : max_results);
+    if (results)
+        for (uint32_t i = 0; i < m; ++i) results[i] = cand[i];
+    if (n_results) *n_results = (uint32_t)cand.size();
+    return FELOG_OK;
+}
+
 felog_status felog_corpus_get(felog_ctx *c, felog_corpus *out) {
     felog_status s = need_built(c);
     if (s) return s;

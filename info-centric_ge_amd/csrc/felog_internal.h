@@ -125,6 +125,9 @@ struct felog_ctx {
     // SGNS tile descriptors (written by the prep kernel, read by the tile kernel)
     void *d_tiles = nullptr;
     uint64_t tiles_cap = 0;
+    // SGNS tile shape (0: default): input rows 48/64, waves per SIMD the
+    // kernel is compiled for, persistent workgroups per CU
+    uint32_t tile_rows = 0, tile_wpe = 0, tile_gpc = 0;
 
     // scratch for library primitives
     void *d_tmp = nullptr;
@@ -234,6 +237,10 @@ uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative);
 void drop_round_graph(felog_ctx *c);
 // Input rows of an SGNS tile: 48 when a group's windows fit, else 64.
 uint32_t sgns_tile_rows(uint32_t centres, uint32_t window);
+// Tile kernel variants: waves per SIMD lo and lo + 1, lo = 2 (FP32) or 3
+// (BF16); workgroups of a shape that fit on one CU.
+uint32_t tile_wpe_low(const felog_ctx *c);
+hipError_t tile_occupancy(felog_ctx *c, uint32_t rows, uint32_t wpe, int *per_cu);
 
 // Release device memory; a failure here has nothing left to report to.
 inline void dfree(void *p) {

@@ -194,8 +194,10 @@ __device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, 
     }
 }
 
-template <typename T, int RM, bool WB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 4))) void k_sgns_tile(Params a) {
+// WPE: waves per SIMD the register budget is cut for (the LDS of one
+// workgroup already limits FP32 tiles to three per CU, BF16 tiles to four).
+template <typename T, int RM, bool WB, int WPE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sgns_tile(Params a) {
     constexpr uint32_t GT = gt_pitch(sizeof(T), RM);  // pitch of the count / G^T tile (words)
     constexpr uint32_t EPV = 16 / sizeof(T);          // elements per 16-byte vector
     constexpr int NB = sizeof(T) == 4 ? 8 : 5;        // gathered vectors held per thread
@@ -609,24 +611,55 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
     if (lane == 0) atomicAdd(&a.counters[C_PAIRS], my_pairs);
 }
 
-template <typename T, int RM, bool WB>
-hipError_t launch(felog_ctx *c, const Params &a) {
+template <typename T, int RM, bool WB, int WPE>
+hipError_t occupancy(felog_ctx *c, int *per_cu) {
     const Lds L = lds_layout(c->dp, sizeof(T), RM);
-    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB>,
+    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total);
     if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE>, kBlock, L.total);
+    if (*per_cu < 1) *per_cu = 1;
+    return e;
+}
+
+// Persistent grid: the tile shape's workgroups per CU (at most what fits).
+template <typename T, int RM, bool WB, int WPE>
+hipError_t launch(felog_ctx *c, const Params &a) {
     int per_cu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sgns_tile<T, RM, WB>, kBlock, L.total);
+    hipError_t e = occupancy<T, RM, WB, WPE>(c, &per_cu);
     if (e != hipSuccess) return e;
-    if (per_cu < 1) per_cu = 1;
+    if (c->tile_gpc && (int)c->tile_gpc < per_cu) per_cu = (int)c->tile_gpc;
     const unsigned blocks = (unsigned)per_cu * c->cu_count;
-    k_sgns_tile<T, RM, WB><<<blocks, kBlock, L.total, c->stream>>>(a);
+    const Lds L = lds_layout(c->dp, sizeof(T), RM);
+    k_sgns_tile<T, RM, WB, WPE><<<blocks, kBlock, L.total, c->stream>>>(a);
     return hipGetLastError();
+}
+
+template <typename T> constexpr int kWpeLow = sizeof(T) == 4 ? 2 : 3;
+
+template <typename T, int RM, bool WB>
+hipError_t launch_wpe(felog_ctx *c, const Params &a) {
+    constexpr int lo = kWpeLow<T>;
+    return c->tile_wpe == (uint32_t)lo ? launch<T, RM, WB, lo>(c, a) : launch<T, RM, WB, lo + 1>(c, a);
 }
 
 template <typename T, int RM>
 hipError_t launch_rm(felog_ctx *c, const Params &a) {
-    return a.writeback ? launch<T, RM, true>(c, a) : launch<T, RM, false>(c, a);
+    return a.writeback ? launch_wpe<T, RM, true>(c, a) : launch_wpe<T, RM, false>(c, a);
+}
+
+template <typename T>
+hipError_t occupancy_t(felog_ctx *c, uint32_t rows, uint32_t wpe, int *per_cu) {
+    constexpr int lo = kWpeLow<T>;
+    const bool wb = c->opt.update_mode != FELOG_UPDATE_ATOMIC;
+    if (rows == 48) {
+        if (wpe == (uint32_t)lo)
+            return wb ? occupancy<T, 48, true, lo>(c, per_cu) : occupancy<T, 48, false, lo>(c, per_cu);
+        return wb ? occupancy<T, 48, true, lo + 1>(c, per_cu) : occupancy<T, 48, false, lo + 1>(c, per_cu);
+    }
+    if (wpe == (uint32_t)lo)
+        return wb ? occupancy<T, 64, true, lo>(c, per_cu) : occupancy<T, 64, false, lo>(c, per_cu);
+    return wb ? occupancy<T, 64, true, lo + 1>(c, per_cu) : occupancy<T, 64, false, lo + 1>(c, per_cu);
 }
 
 }  // namespace
@@ -638,4 +671,12 @@ hipError_t launch_tile(felog_ctx *c, const Params &a) {
 }
 
 }  // namespace sgns
+
+uint32_t tile_wpe_low(const felog_ctx *c) { return c->opt.precision == FELOG_BF16 ? 3u : 2u; }
+
+hipError_t tile_occupancy(felog_ctx *c, uint32_t rows, uint32_t wpe, int *per_cu) {
+    if (c->opt.precision == FELOG_BF16) return sgns::occupancy_t<uint16_t>(c, rows, wpe, per_cu);
+    return sgns::occupancy_t<float>(c, rows, wpe, per_cu);
+}
+
 }  // namespace felog

@@ -78,6 +78,13 @@ class BudgetModel(C.Structure):
                 ("targ_mean", C.c_float * 2), ("targ_std", C.c_float * 2)]
 
 
+class TileShape(C.Structure):
+    """felog_tile_shape: SGNS tile rows, waves per SIMD, workgroups per CU
+    and (from the autotuner) the measured tile-kernel time."""
+    _fields_ = [("rows", C.c_uint32), ("waves_per_simd", C.c_uint32), ("groups_per_cu", C.c_uint32),
+                ("tile_ms", C.c_float)]
+
+
 _lib = None
 
 
@@ -130,6 +137,10 @@ def load_library(path=LIB_PATH):
         "felog_run_budget": (C.c_int32, [P]),
         "felog_run_graph": (C.c_int32, [P, C.c_uint32]),
         "felog_corpus_get": (C.c_int32, [P, C.POINTER(Corpus)]),
+        "felog_tile_shape_set": (C.c_int32, [P, C.POINTER(TileShape)]),
+        "felog_tile_shape_get": (C.c_int32, [P, C.POINTER(TileShape)]),
+        "felog_tune_sgns": (C.c_int32, [P, C.c_uint64, C.c_uint32, C.POINTER(TileShape), C.c_uint32,
+                                        C.POINTER(C.c_uint32)]),
         "felog_visits_reset": (C.c_int32, [P]),
         "felog_visit_stats": (C.c_int32, [P, P]),
     }
@@ -281,6 +292,31 @@ class Felog:
         """run() / run_budget() with each round replayed from a hipGraph;
         at most max_rounds more rounds (0: until the stop rule)."""
         self._check(self.lib.felog_run_graph(self.ctx, max_rounds), "felog_run_graph")
+
+    # -- SGNS tile shape ---------------------------------------------------
+    @staticmethod
+    def _shape_dict(t):
+        return {"rows": t.rows, "waves_per_simd": t.waves_per_simd, "groups_per_cu": t.groups_per_cu,
+                "tile_ms": t.tile_ms}
+
+    def tile_shape(self):
+        t = TileShape()
+        self._check(self.lib.felog_tile_shape_get(self.ctx, C.byref(t)), "felog_tile_shape_get")
+        return self._shape_dict(t)
+
+    def set_tile_shape(self, rows=0, waves_per_simd=0, groups_per_cu=0):
+        """Zero fields take the defaults; set_tile_shape() restores all."""
+        t = TileShape(rows, waves_per_simd, groups_per_cu, 0.0)
+        self._check(self.lib.felog_tile_shape_set(self.ctx, C.byref(t)), "felog_tile_shape_set")
+
+    def tune_sgns(self, sample=0, reps=2):
+        """Time every tile shape on the current walk corpus (first `sample`
+        sentences, 0: all) and install the fastest; returns every shape
+        tried with its mean tile-kernel time."""
+        res = (TileShape * 64)()
+        n = C.c_uint32()
+        self._check(self.lib.felog_tune_sgns(self.ctx, sample, reps, res, 64, C.byref(n)), "felog_tune_sgns")
+        return [self._shape_dict(res[i]) for i in range(min(n.value, 64))]
 
     def visits_reset(self):
         self._check(self.lib.felog_visits_reset(self.ctx), "felog_visits_reset")

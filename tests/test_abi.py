@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     lib = felog.load_library()
     names = felog.header_symbols()
-    assert len(names) == 34
+    assert len(names) == 37
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     out = subprocess.run(["nm", "-D", "--defined-only", felog.LIB_PATH], capture_output=True, text=True).stdout
@@ -45,6 +45,7 @@ def test_options_struct_layout_matches_header():
     assert ctypes.sizeof(felog.Options) == 104
     assert felog.Options.seed.offset == 72 and felog.Options.device.offset == 92
     assert felog.Options.update_mode.offset == 96
+    assert ctypes.sizeof(felog.TileShape) == 16
     assert ctypes.sizeof(felog.Stats) == 8 * 3 + 4 * 2 + 8 * 7 + 8 * 11 + 8
 
 
