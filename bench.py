@@ -49,6 +49,8 @@ WORKLOADS = {
     # name: (generator, vertices, undirected edges, weight law)
     "chunglu-2.24M": ("chung_lu", 2_240_000, 35_000_000, "uniform"),
     "rmat-1.1M": ("rmat", 1_100_000, 3_000_000, "uniform"),
+    # uneven-batching stress: degree max > 30k, lognormal(0, 1) weights
+    "rmat-3.07M": ("rmat", 3_070_000, 117_000_000, "lognormal"),
     "er-10k": ("erdos_renyi", 10_000, 50_000, "uniform"),
 }
 
@@ -69,7 +71,7 @@ def measured_traffic(args):
     """HBM bytes per launch from the committed rocprofv3 counter summary of
     this workload (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy
     calibration), or {} when there is none for it."""
-    tag = {"rmat-1.1M": "rmat1.1M", "chunglu-2.24M": "chunglu2.24M"}.get(args.workload)
+    tag = {"rmat-1.1M": "rmat1.1M", "chunglu-2.24M": "chunglu2.24M", "rmat-3.07M": "rmat3.07M"}.get(args.workload)
     path = os.path.join(ROOT, "profiles", f"traffic_{tag}_{'f32' if args.precision == 'fp32' else 'bf16'}.json")
     if tag is None or args.dim != 128 or not os.path.exists(path):
         return {}

@@ -89,6 +89,12 @@ typedef struct felog_options {
     uint32_t max_rounds;       /* felog_run() round limit (0 -> no limit)    */
     int32_t device;            /* HIP device ordinal                         */
     int32_t update_mode;       /* SGNS row updates: FELOG_UPDATE_*           */
+    int32_t compat_walk;       /* 1: the reference's transition - uniform
+                                  neighbour, accepted with tanh of the degree
+                                  ratio over (deg - common neighbours), else a
+                                  step there and back; common neighbours =
+                                  (int) edge data (include/walk.hpp:976-1019).
+                                  0: weight-proportional alias step          */
 } felog_options;
 
 /* How SGNS tiles write their row deltas (Hogwild either way).

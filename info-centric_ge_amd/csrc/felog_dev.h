@@ -100,6 +100,59 @@ __device__ __forceinline__ bool walk_stats_step(WalkStats &st, const double *__r
     return step > min_L && (R * R < kRThreshold || R < 0.0);
 }
 
+// ---------------------------------------------------------------------------
+// The reference's transition (include/walk.hpp:976-1019): a uniform
+// neighbour x of v, accepted with probability tanh(p) where
+//   p = (1 / (deg v - cn)) * max(deg v, deg x) / min(deg v, deg x)
+// and cn = (int) edge data (the pair's common neighbours); otherwise the
+// walker steps to x and back to v (a "trace").  A library tanh differs
+// between the host libm and the device, and one ulp can flip a decision, so
+// both sides evaluate the restatement below: only + - * / and exact
+// scaling, contraction off.
+//   e^y - 1 = y (1 + y/2 (1 + y/3 (1 + ...)))   for |y| <= 0.5 (to y^18/18!)
+//   e^y     = 2^k (1 + (e^r - 1)),  r = y - k ln2 (fdlibm's two-part ln2)
+//   tanh x  = t / (t + 2),  t = e^{2x} - 1;  |x| >= 20 gives +-1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double expm1_series(double y) {
+#pragma clang fp contract(off)
+    double p = 1.0;
+    for (int i = 18; i >= 2; --i) p = 1.0 + (y / (double)i) * p;
+    return y * p;
+}
+
+__device__ __forceinline__ double tanh_restated(double x) {
+#pragma clang fp contract(off)
+    const bool neg = x < 0.0;
+    const double a = neg ? -x : x;
+    if (!(a < 20.0)) return neg ? -1.0 : 1.0;
+    const double y = a + a;
+    double t;
+    if (y <= 0.5) {
+        t = expm1_series(y);
+    } else {
+        const double k = floor(y * 1.4426950408889634 + 0.5);
+        const double r = (y - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+        t = ldexp(1.0 + expm1_series(r), (int)k) - 1.0;
+    }
+    const double th = t / (t + 2.0);
+    return neg ? -th : th;
+}
+
+// Uniform double in [0, 1) from two draws (53 bits).
+__device__ __forceinline__ double uniform53(uint32_t hi, uint32_t lo) {
+    return (double)((((uint64_t)hi) << 21) | (lo >> 11)) * 0x1.0p-53;
+}
+
+// 1 when the reference's rule traces (tanh(p) <= u) for the step from v
+// (degree dv) to x (degree dx) over an edge with data `data`.
+__device__ __forceinline__ bool cn_trace(uint32_t dv, uint32_t dx, float data, double u) {
+#pragma clang fp contract(off)
+    const int64_t cn = (int64_t)(int)data;
+    const double hi = (double)(dv > dx ? dv : dx), lo = (double)(dv < dx ? dv : dx);
+    const double p = ((1.0 / (double)((int64_t)dv - cn)) * hi) / lo;
+    return tanh_restated(p) <= u;
+}
+
 // bf16 helpers (round-to-nearest-even is never used for table updates; they
 // are stochastically rounded).
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {

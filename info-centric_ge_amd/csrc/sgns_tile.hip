@@ -300,17 +300,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         uint4 buf[NB];
         uint32_t tix[NB], ce[NB];
         uint32_t row = st_r0, c = st_c0;
+        // select-only index math (no short-circuit branches): rr wraps for
+        // input rows, so rr < n_out holds exactly for live output rows
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const bool live = row < n_in || (row >= R16 && row < R16 + n_out);
-            tix[j] = live ? (row < R16 ? row : row + (RMAX - R16)) : 0xFFFFFFFFu;
+            const uint32_t rr = row - R16;
+            const bool live = (row < n_in) | (rr < n_out);
+            tix[j] = live ? (row < R16 ? row : rr + (uint32_t)RMAX) : 0xFFFFFFFFu;
             ce[j] = c * EPV;
             advance(row, c);
         }
         uint32_t grow[NB];
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-            grow[j] = tix[j] != 0xFFFFFFFFu ? toks[tix[j]] + (tix[j] >= (uint32_t)RMAX ? a.V : 0u) : 0u;
+        for (int j = 0; j < NB; ++j) {
+            const bool live = tix[j] != 0xFFFFFFFFu;
+            const uint32_t id = toks[live ? tix[j] : 0u];
+            grow[j] = live ? id + (tix[j] >= (uint32_t)RMAX ? a.V : 0u) : 0u;
+        }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             buf[j] = make_uint4(0, 0, 0, 0);

@@ -44,6 +44,7 @@ struct WalkArgs {
     const double *__restrict__ T;
     const double *__restrict__ dT;
     const uint16_t *__restrict__ fixed_len;  // per start vertex, 0: stop rule (null: none)
+    const float *__restrict__ cn_data;       // edge data: the reference's transition (null: alias)
     uint32_t min_L, max_len;
     uint64_t seed;
 };
@@ -111,37 +112,48 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                 const u32x4 r = draw(a.seed, s, DOM_WALK | attempt, w, round);
                 const uint32_t j = alias_bucket(r.x, vr.deg);
                 const EdgeRec e = a.erec[vr.begin + j];
-                const uint32_t x = (r.y < e.thresh) ? e.col : e.alias_col;
+                // alias step, or the reference's uniform neighbour (cn_data)
+                const uint32_t x = (a.cn_data || r.y < e.thresh) ? e.col : e.alias_col;
                 if (s == 0 && x == v) {
                     if (++attempt >= kMaxStartAttempts) stop = true;
                 } else {
                     attempt = 0;
-                    // prior visits of x on this walk (stop-rule walks only)
-                    uint32_t f = 0;
-                    if (!lim) {
-                        const uint32_t nl = s + 1 < (uint32_t)kHistLds ? s + 1 : (uint32_t)kHistLds;
-                        for (uint32_t k = 0; k < nl; ++k) f += (H[k * 64] == x) ? 1u : 0u;
-                        for (uint32_t k = kHistLds; k <= s; ++k) f += (path_at(a, w, ob, k) == x) ? 1u : 0u;
-                    }
-                    const uint32_t ns = s + 1;
-                    bool stored = true;
-                    if (ns < a.slot) {
-                        a.slots[(uint64_t)w * a.slot + ns] = x;
-                    } else {
-                        if (ob == kNoOverflow) {
-                            const uint64_t need_n = a.max_len - a.slot;
-                            const uint64_t got = atomicAdd(&a.counters[C_OVF_NEXT], (unsigned long long)need_n);
-                            if (got + need_n <= a.ovf_cap) ob = got;
-                            else { atomicAdd(&a.counters[C_OVF_FAIL], 1ull); stored = false; }
+                    // append vertex y at position s + 1: its prior visits on
+                    // this walk (stop-rule walks only), storage, the stop rule
+                    auto append = [&](uint32_t y) -> bool {
+                        uint32_t f = 0;
+                        if (!lim) {
+                            const uint32_t nl = s + 1 < (uint32_t)kHistLds ? s + 1 : (uint32_t)kHistLds;
+                            for (uint32_t k = 0; k < nl; ++k) f += (H[k * 64] == y) ? 1u : 0u;
+                            for (uint32_t k = kHistLds; k <= s; ++k) f += (path_at(a, w, ob, k) == y) ? 1u : 0u;
                         }
-                        if (stored) a.ovf[ob + (ns - a.slot)] = x;
-                    }
-                    if (!stored) {
-                        stop = true;  // truncated; the host reports FELOG_ERR_CAPACITY
-                    } else {
-                        if (ns < (uint32_t)kHistLds) H[ns * 64] = x;
+                        const uint32_t ns = s + 1;
+                        if (ns < a.slot) {
+                            a.slots[(uint64_t)w * a.slot + ns] = y;
+                        } else {
+                            if (ob == kNoOverflow) {
+                                const uint64_t need_n = a.max_len - a.slot;
+                                const uint64_t got = atomicAdd(&a.counters[C_OVF_NEXT], (unsigned long long)need_n);
+                                if (got + need_n > a.ovf_cap) {
+                                    atomicAdd(&a.counters[C_OVF_FAIL], 1ull);
+                                    return true;  // truncated; the host reports FELOG_ERR_CAPACITY
+                                }
+                                ob = got;
+                            }
+                            a.ovf[ob + (ns - a.slot)] = y;
+                        }
+                        if (ns < (uint32_t)kHistLds) H[ns * 64] = y;
                         s = ns;
-                        stop = lim ? s + 1 >= lim : walk_stats_step(st, a.T, a.dT, f, s + 1, s, a.min_L);
+                        return lim ? s + 1 >= lim : walk_stats_step(st, a.T, a.dT, f, s + 1, s, a.min_L);
+                    };
+                    bool trace = false;
+                    if (a.cn_data)
+                        trace = cn_trace(vr.deg, a.vrec[x].deg, a.cn_data[vr.begin + j], uniform53(r.y, r.z));
+                    stop = append(x);
+                    if (trace) {
+                        // include/walk.hpp:1063-1094: back to v, which stays current
+                        if (!stop) stop = s + 1 >= a.max_len || append(v);
+                    } else {
                         v = x;
                     }
                 }
@@ -344,6 +356,7 @@ hipError_t walk_enqueue(felog_ctx *c) {
     a.T = c->d_T;
     a.dT = c->d_dT;
     a.fixed_len = c->budget_on ? c->d_budget_len : nullptr;
+    a.cn_data = c->opt.compat_walk ? c->d_w : nullptr;
     a.min_L = c->opt.min_L;
     a.max_len = c->opt.max_walk_len;
     a.seed = c->opt.seed;

@@ -42,7 +42,7 @@ class Options(C.Structure):
         ("min_count", C.c_uint32), ("binary", C.c_int32), ("precision", C.c_int32),
         ("max_walk_len", C.c_uint32), ("overflow_capacity", C.c_uint64), ("seed", C.c_uint64),
         ("compat_lr", C.c_int32), ("compat_vocab_break", C.c_int32), ("max_rounds", C.c_uint32),
-        ("device", C.c_int32), ("update_mode", C.c_int32),
+        ("device", C.c_int32), ("update_mode", C.c_int32), ("compat_walk", C.c_int32),
     ]
 
 

@@ -12,7 +12,7 @@
 //      (ArgPos, src/word2vec.cu:2182-2192,2269-2286).
 // Switches of this build (also found anywhere): --precision fp32|bf16,
 // --seed N, --max-rounds N, --max-walk-len N, --device N, --write-walks,
-// --felog-compat-lr, --felog-compat-vocab.
+// --felog-compat-lr, --felog-compat-vocab, --felog-compat-walk.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -126,6 +126,7 @@ int main(int argc, char **argv) {
     if ((i = arg_pos("--device", argc, argv, true)) > 0) o.device = std::atoi(argv[i + 1]);
     o.compat_lr = arg_pos("--felog-compat-lr", argc, argv, false) > 0;
     o.compat_vocab_break = arg_pos("--felog-compat-vocab", argc, argv, false) > 0;
+    o.compat_walk = arg_pos("--felog-compat-walk", argc, argv, false) > 0;
     const bool write_walks = arg_pos("--write-walks", argc, argv, false) > 0;
 
     if (emb.empty()) {

@@ -243,10 +243,91 @@ static uint32_t walk_one(uint32_t w, uint32_t V, const uint64_t *row_ptr, const 
     return s + 1;
 }
 
+/* The reference's transition (include/walk.hpp:962-1101): a uniform
+ * neighbour x of the current vertex v (Philox word 0, as the alias column),
+ * u = a uniform double from words 1-2, p = (1 / (deg v - cn)) * max(deg v,
+ * deg x) / min(deg v, deg x) with cn = (int) edge data; when tanh(p) <= u
+ * the walker appends x and then v again (a "trace", :1015-1019,1063-1094)
+ * and stays at v, else it moves to x.  Each appended vertex runs the stop
+ * rule.  tanh is restated from + - * / and exact scaling (Taylor series of
+ * e^y - 1 on |y| <= 0.5 to the y^18 term, 2^k scaling with fdlibm's
+ * two-part ln2 beyond), evaluated in the same order as the device so the
+ * decisions agree bit for bit; its accuracy is pinned against libm. */
+static double expm1_series(double y) {
+    double p = 1.0;
+    for (int i = 18; i >= 2; --i) p = 1.0 + (y / (double)i) * p;
+    return y * p;
+}
+
+double orc_tanh_restated(double x) {
+    const int neg = x < 0.0;
+    const double a = neg ? -x : x;
+    if (!(a < 20.0)) return neg ? -1.0 : 1.0;
+    const double y = a + a;
+    double t;
+    if (y <= 0.5) {
+        t = expm1_series(y);
+    } else {
+        const double k = floor(y * 1.4426950408889634 + 0.5);
+        const double r = (y - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+        t = ldexp(1.0 + expm1_series(r), (int)k) - 1.0;
+    }
+    const double th = t / (t + 2.0);
+    return neg ? -th : th;
+}
+
+static uint32_t walk_one_cn(uint32_t w, uint32_t V, const uint64_t *row_ptr, const uint32_t *col,
+                            const float *cn_data, uint32_t min_L, uint32_t max_len, uint32_t round, uint64_t seed,
+                            const double *T, const double *dT, uint32_t *path) {
+    uint32_t v = w % V, s = 0, attempt = 0;
+    stats_t st;
+    stats_init(&st);
+    path[0] = v;
+    for (;;) {
+        if (s + 1 >= max_len) break;
+        const uint64_t b = row_ptr[v];
+        const uint32_t deg = (uint32_t)(row_ptr[v + 1] - b);
+        if (deg == 0) break;
+        if (deg == 2 && col[b] == v) break;
+        uint32_t r[4];
+        rnd(seed, s, D_WALK | attempt, w, round, r);
+        const uint32_t j = mulhi32(r[0], deg);
+        const uint32_t x = col[b + j];
+        if (s == 0 && x == v) {
+            if (++attempt >= 64) break;
+            continue;
+        }
+        attempt = 0;
+        const uint32_t dx = (uint32_t)(row_ptr[x + 1] - row_ptr[x]);
+        const int64_t cn = (int64_t)(int)cn_data[b + j];
+        const double hi = (double)(deg > dx ? deg : dx), lo = (double)(deg < dx ? deg : dx);
+        const double p = ((1.0 / (double)((int64_t)deg - cn)) * hi) / lo;
+        const double u = (double)((((uint64_t)r[1]) << 21) | (r[2] >> 11)) * 0x1.0p-53;
+        const int trace = orc_tanh_restated(p) <= u;
+        int stop = 0;
+        for (int leg = 0; leg < (trace ? 2 : 1) && !stop; ++leg) {
+            const uint32_t y = leg ? v : x;
+            if (leg && s + 1 >= max_len) { stop = 1; break; }
+            uint32_t f = 0;
+            for (uint32_t k = 0; k <= s; ++k) f += path[k] == y;
+            ++s;
+            path[s] = y;
+            double R;
+            stats_step(&st, T, dT, f, s + 1, &R);
+            stop = s > min_L && (R * R < 0.999 || R < 0.0);
+        }
+        if (stop) break;
+        if (!trace) v = x;
+    }
+    return s + 1;
+}
+
+/* Walks of one round; cn_data (edge data per CSR entry) selects the
+ * reference's transition, NULL the alias step. */
 uint64_t orc_walks(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, const uint32_t *thresh,
-                   const uint32_t *alias_idx, const uint8_t *flags, uint32_t walkers, uint32_t min_L,
-                   uint32_t max_len, uint32_t round, uint64_t seed, uint64_t *offsets, uint32_t *tokens,
-                   uint32_t *walker_ids, uint64_t *n_tokens, uint64_t *n_steps, int threads) {
+                   const uint32_t *alias_idx, const float *cn_data, const uint8_t *flags, uint32_t walkers,
+                   uint32_t min_L, uint32_t max_len, uint32_t round, uint64_t seed, uint64_t *offsets,
+                   uint32_t *tokens, uint32_t *walker_ids, uint64_t *n_tokens, uint64_t *n_steps, int threads) {
     double *T = (double *)malloc(((uint64_t)max_len + 2) * sizeof(double));
     double *dT = (double *)malloc(((uint64_t)max_len + 1) * sizeof(double));
     orc_xlog2x(max_len, T, dT);
@@ -264,7 +345,9 @@ uint64_t orc_walks(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, con
         uint32_t *path = (uint32_t *)malloc((uint64_t)max_len * sizeof(uint32_t));
 #pragma omp for schedule(dynamic, 256)
         for (int64_t i = 0; i < (int64_t)n; ++i)
-            lens[i] = walk_one(act[i], V, row_ptr, col, thresh, alias_idx, min_L, max_len, round, seed, T, dT, path);
+            lens[i] = cn_data ? walk_one_cn(act[i], V, row_ptr, col, cn_data, min_L, max_len, round, seed, T, dT, path)
+                              : walk_one(act[i], V, row_ptr, col, thresh, alias_idx, min_L, max_len, round, seed, T,
+                                         dT, path);
         free(path);
     }
     uint64_t tot = 0, steps = 0;
@@ -282,8 +365,10 @@ uint64_t orc_walks(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, con
             uint32_t *path = (uint32_t *)malloc((uint64_t)max_len * sizeof(uint32_t));
 #pragma omp for schedule(dynamic, 256)
             for (int64_t i = 0; i < (int64_t)n; ++i) {
-                const uint32_t len = walk_one(act[i], V, row_ptr, col, thresh, alias_idx, min_L, max_len, round,
-                                              seed, T, dT, path);
+                const uint32_t len =
+                    cn_data ? walk_one_cn(act[i], V, row_ptr, col, cn_data, min_L, max_len, round, seed, T, dT, path)
+                            : walk_one(act[i], V, row_ptr, col, thresh, alias_idx, min_L, max_len, round, seed, T,
+                                       dT, path);
                 memcpy(tokens + offsets[i], path, (uint64_t)len * sizeof(uint32_t));
             }
             free(path);

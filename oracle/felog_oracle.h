@@ -13,7 +13,10 @@
  *   orc_corr_reference /
  *   orc_corr_exact         Shannon entropy and two-pass Pearson recomputed
  *                          from the visit histogram (closed forms)
- *   orc_walks              invariants (edges exist, stop rule) + the above
+ *   orc_walks              invariants (edges exist, stop rule) + the above;
+ *                          compat transition: chi-square of the first step
+ *                          and trace frequencies against closed form
+ *   orc_tanh_restated      libm tanh within a few ulp
  *   orc_init_syn0          the reference's sequential LCG loop, element 0..n
  *   orc_sgns_groups        finite differences of the SGNS loss on d <= 8
  *   orc_w2v_train          single-pair update == hand-computed SGNS step
@@ -53,14 +56,19 @@ void orc_xlog2x(uint32_t L, double *T, double *dT);
 void orc_corr_reference(uint32_t n, const uint32_t *f, double *H, double *R);
 void orc_corr_exact(uint32_t n, const uint32_t *f, const double *T, const double *dT, double *H, double *R);
 
-/* Host replay of one round of alias walks.  Walker w starts at w % V and is
- * skipped when flags[w % V] != 0.  Writes the flat corpus in walker order:
- * offsets[n_walks+1], tokens[...], walker_ids[n_walks].  Returns n_walks;
- * *n_tokens and *n_steps receive the totals.  tokens may be NULL to count. */
+/* Host replay of one round of walks: alias steps, or with cn_data (edge
+ * data per CSR entry) the reference's common-neighbour transition.  Walker
+ * w starts at w % V and is skipped when flags[w % V] != 0.  Writes the flat
+ * corpus in walker order: offsets[n_walks+1], tokens[...],
+ * walker_ids[n_walks].  Returns n_walks; *n_tokens and *n_steps receive the
+ * totals.  tokens may be NULL to count. */
 uint64_t orc_walks(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, const uint32_t *thresh,
-                   const uint32_t *alias_idx, const uint8_t *flags, uint32_t walkers, uint32_t min_L,
-                   uint32_t max_len, uint32_t round, uint64_t seed, uint64_t *offsets, uint32_t *tokens,
-                   uint32_t *walker_ids, uint64_t *n_tokens, uint64_t *n_steps, int threads);
+                   const uint32_t *alias_idx, const float *cn_data, const uint8_t *flags, uint32_t walkers,
+                   uint32_t min_L, uint32_t max_len, uint32_t round, uint64_t seed, uint64_t *offsets,
+                   uint32_t *tokens, uint32_t *walker_ids, uint64_t *n_tokens, uint64_t *n_steps, int threads);
+/* tanh restated from + - * / and exact scaling (the compat transition's
+ * acceptance, include/walk.hpp:1005-1006). */
+double orc_tanh_restated(double x);
 
 /* Vocabulary (src/word2vec.cu:697-755,834-862): order by degree desc, id
  * asc; in_vocab = degree >= min_count or rank 0. */

@@ -24,6 +24,8 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.orc_auc.restype = C.c_double
         _lib.orc_walks.restype = C.c_uint64
+        _lib.orc_tanh_restated.restype = C.c_double
+        _lib.orc_tanh_restated.argtypes = [C.c_double]
         _lib.orc_w2v_train.restype = C.c_uint64
         _lib.orc_eval.restype = C.c_uint32
         _lib.orc_sgns_groups.restype = C.c_uint32
@@ -102,13 +104,17 @@ def corr_exact(f, L=1000):
 
 
 def walks(V, row_ptr, col, thresh, alias_idx, flags, walkers, min_L, max_len, rnd, seed, threads=0,
-          want_tokens=True):
+          want_tokens=True, cn_data=None):
+    """One round of walks.  cn_data (edge data per CSR entry) selects the
+    reference's common-neighbour transition instead of the alias step."""
     flags = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    cn_data = None if cn_data is None else np.ascontiguousarray(cn_data, np.float32)
     L = lib()
     nt = C.c_uint64()
     ns = C.c_uint64()
     args = (C.c_uint32(V), P(np.ascontiguousarray(row_ptr, np.uint64)), P(np.ascontiguousarray(col, np.uint32)),
-            P(np.ascontiguousarray(thresh, np.uint32)), P(np.ascontiguousarray(alias_idx, np.uint32)), P(flags),
+            P(np.ascontiguousarray(thresh, np.uint32)), P(np.ascontiguousarray(alias_idx, np.uint32)), P(cn_data),
+            P(flags),
             C.c_uint32(walkers), C.c_uint32(min_L), C.c_uint32(max_len), C.c_uint32(rnd), C.c_uint64(seed))
     off = np.empty(walkers + 1, np.uint64)
     wid = np.empty(max(walkers, 1), np.uint32)
@@ -118,6 +124,10 @@ def walks(V, row_ptr, col, thresh, alias_idx, flags, walkers, min_L, max_len, rn
     tok = np.empty(max(nt.value, 1), np.uint32)
     n = L.orc_walks(*args, P(off), P(tok), P(wid), C.byref(nt), C.byref(ns), C.c_int(threads))
     return off[:n + 1], tok[:nt.value], wid[:n], int(ns.value)
+
+
+def tanh_restated(x):
+    return lib().orc_tanh_restated(float(x))
 
 
 def vocab(deg, min_count):

@@ -83,6 +83,34 @@ def test_walks_bitexact(er, rnd):
     assert np.array_equal(tok, tok2)
 
 
+@pytest.mark.parametrize("undirected", [False, True])
+def test_compat_walks_bitexact(undirected):
+    """The reference's common-neighbour transition (compat_walk): uniform
+    neighbour, tanh acceptance, trace steps.  Edge data in [0, 4) gives
+    common-neighbour counts 0..3; two rounds against the host replay."""
+    n, m = 10_000, 50_000
+    rng = np.random.default_rng(12)
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    w = (rng.random(m) * 4).astype(np.float32)
+    opts = dict(size=64, window=10, negative=5, min_L=20, min_R=2, seed=SEED, compat_walk=1)
+    print(f"[case] compat walks V={n} m={m} undirected={undirected}", flush=True)
+    f = felog.Felog(**opts)
+    try:
+        f.set_edges(src, dst, w, n, make_undirected=undirected)
+        f.build()
+        rp, col, wo = f.csr()
+        t, a = f.alias()
+        for rnd in (1, 2):
+            c = f.walk_round(rnd)
+            off, tok, wid = f.corpus()
+            off2, tok2, wid2, steps2 = orc.walks(n, rp, col, t, a, None, n, 20, 1000, rnd, SEED, cn_data=wo)
+            assert c["n_walks"] == len(wid2) and c["n_steps"] == steps2
+            assert np.array_equal(wid, wid2) and np.array_equal(off, off2) and np.array_equal(tok, tok2)
+    finally:
+        f.close()
+
+
 def _random_tables(V, d, seed):
     rng = np.random.default_rng(seed)
     return (rng.standard_normal((V, d)) * 0.3).astype(np.float32), (rng.standard_normal((V, d)) * 0.3).astype(np.float32)

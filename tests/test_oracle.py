@@ -213,6 +213,100 @@ def test_walk_edge_cases():
     assert np.diff(off.astype(np.int64))[0] == 7
 
 
+# ---- the reference's common-neighbour transition (compat mode)
+def test_tanh_restated_matches_libm():
+    xs = np.concatenate([np.logspace(-12, np.log10(19.99), 3000), [0.0, 0.25, 0.2500001, 19.999, 20.0, 25.0, 1e300]])
+    for x in np.concatenate([xs, -xs]):
+        got, ref = orc.tanh_restated(x), math.tanh(x)
+        if abs(x) >= 20.0:
+            assert got == math.copysign(1.0, x)
+        else:
+            assert abs(got - ref) <= 4 * np.spacing(abs(ref)), (x, got, ref)
+
+
+def _directed_star():
+    """Vertex 0 -> 1..4 with edge data 0.5, 1.5, 2.5, 0.25 (common
+    neighbours 0, 1, 2, 0); out-degrees of 1..4 are 1, 2, 3, 6 and none
+    leads back to 0, so a walk revisits 0 at step 2 only by a trace."""
+    src, dst, w = [0, 0, 0, 0], [1, 2, 3, 4], [0.5, 1.5, 2.5, 0.25]
+    outs = {1: [5], 2: [5, 6], 3: [5, 6, 7], 4: [5, 6, 7, 8, 9, 10]}
+    for v, us in outs.items():
+        src += [v] * len(us)
+        dst += us
+        w += [1.0] * len(us)
+    for v in range(5, 11):  # sinks walk on among themselves
+        src += [v, v]
+        dst += [5 + (v - 4) % 6, 5 + (v - 3) % 6]
+        w += [1.0, 1.0]
+    return (np.array(src, np.uint32), np.array(dst, np.uint32), np.array(w, np.float32), 11)
+
+
+def test_compat_walk_first_step_matches_closed_form():
+    """From vertex 0 (out-degree 4): the neighbour is uniform, and the walk
+    traces back with probability 1 - tanh((1 / (4 - cn)) * max(4, d) / min(4, d))
+    (include/walk.hpp:997-1019), tanh from libm here."""
+    s, d, w, V = _directed_star()
+    rp, col, wo = orc.build_csr(V, s, d, w, False)
+    t, a = orc.build_alias(V, rp, wo)
+    M = 40000
+    flags = np.ones(V, np.uint8)
+    flags[0] = 0
+    off, tok, wid, _ = orc.walks(V, rp, col, t, a, flags, V * M, 20, 1000, 1, 77, cn_data=wo)
+    assert len(wid) == M
+    starts = off[:-1].astype(np.int64)
+    first, second = tok[starts + 1], tok[starts + 2]
+    deg = np.diff(rp.astype(np.int64))
+    cn = {1: 0, 2: 1, 3: 2, 4: 0}
+    counts = np.array([(first == j).sum() for j in (1, 2, 3, 4)])
+    assert counts.sum() == M
+    chi2 = ((counts - M / 4) ** 2 / (M / 4)).sum()
+    assert chi2 < 16.3  # 3 degrees of freedom, p = 0.001
+    for j in (1, 2, 3, 4):
+        p = (1.0 / (4 - cn[j])) * max(4, deg[j]) / min(4, deg[j])
+        q = 1.0 - math.tanh(p)
+        n = (first == j).sum()
+        k = (second[first == j] == 0).sum()
+        sd = math.sqrt(q * (1 - q) / n)
+        assert abs(k / n - q) < 4.5 * sd, (j, k / n, q)
+
+
+def test_compat_walks_follow_edges_or_trace_and_stop_rule():
+    """Random directed graph: every step is an edge or a trace back to the
+    vertex two before; the stop rule ends each walk at its first allowed
+    step; replays are thread-count independent."""
+    rng = np.random.default_rng(8)
+    V, m = 1500, 9000
+    src = rng.integers(0, V, m).astype(np.uint32)
+    dst = rng.integers(0, V, m).astype(np.uint32)
+    w = (rng.random(m) * 4).astype(np.float32)
+    rp, col, wo = orc.build_csr(V, src, dst, w, False)
+    t, a = orc.build_alias(V, rp, wo)
+    off, tok, wid, _ = orc.walks(V, rp, col, t, a, None, V, 20, 1000, 1, 3, threads=4, cn_data=wo)
+    off1, tok1, wid1, _ = orc.walks(V, rp, col, t, a, None, V, 20, 1000, 1, 3, threads=1, cn_data=wo)
+    assert np.array_equal(tok, tok1) and np.array_equal(off, off1)
+    edges = set()
+    for v in range(V):
+        for u in col[rp[v]:rp[v + 1]]:
+            edges.add((v, int(u)))
+    traces = 0
+    for i in range(0, len(wid), 7):
+        p = tok[off[i]:off[i + 1]].tolist()
+        for j in range(len(p) - 1):
+            if (p[j], p[j + 1]) not in edges:
+                assert j >= 1 and p[j + 1] == p[j - 1], (i, j)
+                traces += 1
+        if len(p) > 1 and deg_ok(rp, p[-1]):
+            _, R = closed_form_series(p)
+            stops = [s for s in range(21, len(p)) if R[s - 1] ** 2 < 0.999 or R[s - 1] < 0]
+            assert stops and stops[0] == len(p) - 1
+    assert traces > 0
+
+
+def deg_ok(rp, v):
+    """The walk ended by the stop rule (not by a degree-0 vertex)."""
+    return rp[v + 1] > rp[v]
+
+
 # ---- vocabulary, thresholds, initialisation
 def test_vocab_order_and_thresholds():
     rng = np.random.default_rng(2)
