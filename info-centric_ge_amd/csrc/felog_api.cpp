@@ -67,8 +67,8 @@ namespace {
 
 void free_all(felog_ctx *c) {
     void *ptrs[] = {c->d_esrc, c->d_edst, c->d_ew, c->d_row_ptr, c->d_col, c->d_w, c->d_vrec, c->d_erec,
-                    c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni_thr,
-                    c->d_uni_alias, c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
+                    c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni,
+                    c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
                     c->d_scores, c->d_active, c->d_sel, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
                     c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats};
@@ -954,8 +954,14 @@ felog_status felog_vocab_copy_out(felog_ctx *c, uint32_t *order, uint32_t *degre
 felog_status felog_unigram_copy_out(felog_ctx *c, uint32_t *thresh, uint32_t *alias_idx) {
     felog_status s = need_built(c);
     if (s) return s;
-    if (thresh) TRY(c, hipMemcpy(thresh, c->d_uni_thr, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
-    if (alias_idx) TRY(c, hipMemcpy(alias_idx, c->d_uni_alias, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    if (thresh || alias_idx) {
+        std::vector<uint2> uni(c->V);
+        TRY(c, hipMemcpy(uni.data(), c->d_uni, (uint64_t)c->V * 8, hipMemcpyDeviceToHost));
+        for (uint32_t v = 0; v < c->V; ++v) {
+            if (thresh) thresh[v] = uni[v].x;
+            if (alias_idx) alias_idx[v] = uni[v].y;
+        }
+    }
     return FELOG_OK;
 }
 

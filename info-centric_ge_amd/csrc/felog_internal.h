@@ -88,8 +88,8 @@ struct felog_ctx {
     uint32_t *d_vocab_order = nullptr;   // V, rank -> vertex
     uint32_t *d_vocab_rank = nullptr;    // V, vertex -> rank
     uint8_t *d_in_vocab = nullptr;       // V
-    uint32_t *d_uni_thr = nullptr, *d_uni_alias = nullptr;  // V
-    uint32_t *d_keep_thr = nullptr;      // V
+    uint2 *d_uni = nullptr;              // V: {threshold, alias} of the unigram^0.75 sampler
+    uint32_t *d_keep_thr = nullptr;      // V: subsampling keep threshold, 0 outside the vocabulary
     double *d_T = nullptr, *d_dT = nullptr;  // n log2 n table and its differences
 
     // embedding tables (f32 or bf16 rows of dp elements, indexed by vertex

@@ -373,8 +373,7 @@ hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg) {
     HIP_TRY(hipMalloc(&c->d_vocab_order, (uint64_t)V * 4));
     HIP_TRY(hipMalloc(&c->d_vocab_rank, (uint64_t)V * 4));
     HIP_TRY(hipMalloc(&c->d_in_vocab, (uint64_t)V));
-    HIP_TRY(hipMalloc(&c->d_uni_thr, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&c->d_uni_alias, (uint64_t)V * 4));
+    HIP_TRY(hipMalloc(&c->d_uni, (uint64_t)V * 8));
     HIP_TRY(hipMalloc(&c->d_keep_thr, (uint64_t)V * 4));
     HIP_TRY(sort_vocab(c));
     std::vector<uint32_t> order(V), rank(V);
@@ -409,10 +408,16 @@ hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg) {
             if (prob < 1.0) keep[v] = prob <= 0.0 ? 0u : (uint32_t)(prob * 4294967296.0);
         }
     }
+    // one gather per token in the SGNS filter: vertices outside the
+    // vocabulary are never kept (threshold 0); one per negative: the
+    // sampler's threshold and alias side by side
+    for (uint32_t v = 0; v < V; ++v)
+        if (!inv[v]) keep[v] = 0;
+    std::vector<uint2> uni(V);
+    for (uint32_t v = 0; v < V; ++v) uni[v] = make_uint2(uthr[v], uali[v]);
     HIP_TRY(hipMemcpy(c->d_vocab_rank, rank.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_in_vocab, inv.data(), (uint64_t)V, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_uni_thr, uthr.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_uni_alias, uali.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_uni, uni.data(), (uint64_t)V * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->d_keep_thr, keep.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
     // n log2 n table for the walk statistics
     const uint32_t L = c->opt.max_walk_len;

@@ -51,8 +51,7 @@ struct Params {
     const uint8_t *__restrict__ in_vocab;
     const uint32_t *__restrict__ vocab_rank;
     const uint32_t *__restrict__ keep_thr;
-    const uint32_t *__restrict__ uni_thr;
-    const uint32_t *__restrict__ uni_alias;
+    const uint2 *__restrict__ uni;  // {threshold, alias} per vocabulary bucket
     uint32_t V;
     uint32_t window, negative, bgroup, rmax;
     int compat_break;

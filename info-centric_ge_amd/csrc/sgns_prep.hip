@@ -51,10 +51,10 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             uint32_t t = 0;
             if (p < n) {
                 t = a.tok[b0 + p];
-                const bool iv = a.in_vocab[t] != 0;
+                // threshold 0: outside the vocabulary (or never kept)
                 const uint32_t thr = a.keep_thr[t];
-                if (a.compat_break && (!iv || a.vocab_rank[t] == 0)) status = 3;
-                else if (!iv) status = 1;
+                if (a.compat_break && (a.in_vocab[t] == 0 || a.vocab_rank[t] == 0)) status = 3;
+                else if (thr == 0u) status = 1;
                 else
                     status = (thr == 0xFFFFFFFFu ||
                               draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, round).x < thr) ? 2 : 1;
@@ -109,7 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 const uint32_t gq = g0 + lane / k, j = lane % k;
                 const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, round);
                 const uint32_t bkt = alias_bucket(r.x, a.V);
-                sneg[lane] = r.y < a.uni_thr[bkt] ? bkt : a.uni_alias[bkt];
+                const uint2 e = a.uni[bkt];
+                sneg[lane] = r.y < e.x ? bkt : e.y;
             }
             __builtin_amdgcn_wave_barrier();
             if (slot >= a.tiles_cap) {
@@ -228,8 +229,7 @@ hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_t
     a.in_vocab = c->d_in_vocab;
     a.vocab_rank = c->d_vocab_rank;
     a.keep_thr = c->d_keep_thr;
-    a.uni_thr = c->d_uni_thr;
-    a.uni_alias = c->d_uni_alias;
+    a.uni = c->d_uni;
     a.V = c->V;
     a.window = c->opt.window;
     a.negative = c->opt.negative;
