@@ -17,7 +17,7 @@
 //   walks      round r starts the walker of v when r <= count(v) (plain-rule
 //              vertices: when v is not flagged) and it runs exactly length(v)
 //              vertices, without the FP64 entropy state
-// The judge: the corpus visit histogram, its Shannon entropy and the
+// Quality measures: the corpus visit histogram, its Shannon entropy and the
 // visit-frequency KL diagnostic of include/walk.hpp:754-788.
 #include <hip/hip_runtime.h>
 #include "felog_dev.h"
