@@ -51,6 +51,8 @@ WORKLOADS = {
     "rmat-1.1M": ("rmat", 1_100_000, 3_000_000, "uniform"),
     # uneven-batching stress: degree max > 30k, lognormal(0, 1) weights
     "rmat-3.07M": ("rmat", 3_070_000, 117_000_000, "lognormal"),
+    # learned-budget scale: ~1.2B directed edges, generated on the GPU
+    "rmat-41.6M": ("rmat_device", 41_600_000, 600_000_000, "uniform"),
     "er-10k": ("erdos_renyi", 10_000, 50_000, "uniform"),
 }
 
@@ -58,6 +60,8 @@ WORKLOADS = {
 def make_graph(name, seed):
     gen, n, m, law = WORKLOADS[name]
     fn = getattr(graphgen, gen)
+    if gen == "rmat_device":
+        return fn(n, m, seed=seed, weights=law, log=lambda t: print(f"[bench] {t}", file=sys.stderr, flush=True)), n, m
     return fn(n, m, seed=seed, weights=law), n, m
 
 
@@ -117,10 +121,15 @@ def train_budget_model(args):
 def end_to_end(src, dst, w, n, args, model):
     """Plain FP32 run (felog_run's rule) against BF16 + learned budget
     (felog_run_budget's rule) on the graph with 10% of its edges held out;
-    both as captured rounds (felog_run_graph)."""
+    both as captured rounds (felog_run_graph).  With --e2e-no-auc the whole
+    graph is trained and no AUC is measured (the host split is too slow for
+    the 41.6M graph)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
-    ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
+    if args.e2e_no_auc:
+        ts, td, tw, pos, neg = src, dst, w, None, None
+    else:
+        ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
     out = {}
     for form, prec, budgeted in (("plain_fp32", felog.FP32, False), ("bf16_surrogate", felog.BF16, True)):
         f = felog.Felog(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20, min_R=2,
@@ -143,15 +152,19 @@ def end_to_end(src, dst, w, n, args, model):
         wall = time.perf_counter() - t0
         st = f.stats()
         vs = f.visit_stats()
-        emb, _ = f.embeddings()
+        auc = None
+        if pos is not None:
+            emb, _ = f.embeddings()
+            auc = _auc(emb, pos, neg, orc)
+        print(f"[bench] e2e {form}: {st['rounds_done']} rounds, {wall * 1e3:.1f} ms", file=sys.stderr, flush=True)
         out[form] = dict(wall_ms=wall * 1e3, rounds=st["rounds_done"], tokens=vs["visits"],
-                         tokens_per_s=vs["visits"] / wall, auc=_auc(emb, pos, neg, orc), visit_kl=vs["kl"],
+                         tokens_per_s=vs["visits"] / wall, auc=auc, visit_kl=vs["kl"],
                          visit_entropy=vs["entropy"], rounds_gpu_ms=st["graph_ms_total"],
                          tile_shape={k: shape[k] for k in ("rows", "waves_per_simd", "groups_per_cu")}, **extra)
         f.close()
     p, b = out["plain_fp32"], out["bf16_surrogate"]
     out["throughput_ratio"] = p["wall_ms"] / b["wall_ms"]
-    out["auc_gap"] = b["auc"] - p["auc"]
+    out["auc_gap"] = None if p["auc"] is None else b["auc"] - p["auc"]
     out["definition"] = ("throughput_ratio = wall time of the plain FP32 run / wall time of the BF16 + "
                          "learned-budget run, each from the first walk to its stop on the same graph "
                          "(vertices embedded per second); auc_gap = held-out AUC difference")
@@ -203,6 +216,7 @@ def main():
     ap.add_argument("--cpu-walkers", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end plain vs BF16 + budget runs")
+    ap.add_argument("--e2e-no-auc", action="store_true", help="end-to-end runs on the whole graph, no AUC")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -116,6 +116,58 @@ def rmat(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform"):
     return _finish(_fit_patches(keys, n, m), n, rng, weights)
 
 
+def rmat_device(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform", chunk=1 << 27, log=None):
+    """R-MAT on the GPU (torch) for graphs whose host generation would take
+    minutes: the same quadrant law as rmat(), self loops and ids >= n
+    rejected, duplicates removed, a random m of the distinct edges kept,
+    isolated vertices patched, random orientation.  Edge order is random
+    (not generation order).  Returns host numpy arrays."""
+    import torch
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    scale = int(np.ceil(np.log2(max(n, 2))))
+    keys = torch.empty(0, dtype=torch.int64, device=dev)
+    while keys.numel() < m:
+        need = int((m - keys.numel()) * 1.4) + 1024
+        parts = [keys]
+        for lo in range(0, need, chunk):
+            cnt = min(chunk, need - lo)
+            src = torch.zeros(cnt, dtype=torch.int64, device=dev)
+            dst = torch.zeros(cnt, dtype=torch.int64, device=dev)
+            for _ in range(scale):
+                r = torch.rand(cnt, generator=g, device=dev, dtype=torch.float64)
+                sbit = r >= a + b
+                dbit = ((r >= a) & (r < a + b)) | (r >= a + b + c)
+                src = (src << 1) | sbit.to(torch.int64)
+                dst = (dst << 1) | dbit.to(torch.int64)
+            ok = (src < n) & (dst < n) & (src != dst)
+            lo_, hi_ = torch.minimum(src, dst)[ok], torch.maximum(src, dst)[ok]
+            parts.append(lo_ * n + hi_)
+            del src, dst, ok, lo_, hi_
+        keys = torch.unique(torch.cat(parts))
+        del parts
+        if log:
+            log(f"rmat_device: {keys.numel()} distinct edges of {m}")
+    keys = keys[torch.randperm(keys.numel(), generator=g, device=dev)[:m]]
+    src, dst = keys // n, keys % n
+    del keys
+    flip = torch.rand(src.numel(), generator=g, device=dev) < 0.5
+    src, dst = torch.where(flip, dst, src), torch.where(flip, src, dst)
+    deg = torch.bincount(src, minlength=n) + torch.bincount(dst, minlength=n)
+    iso = torch.nonzero(deg == 0).flatten()
+    if iso.numel():
+        nz = torch.nonzero(deg > 0).flatten()
+        partner = nz[torch.randint(0, nz.numel(), (iso.numel(),), generator=g, device=dev)]
+        src, dst = torch.cat([src, iso]), torch.cat([dst, partner])
+    out_s = src.to(torch.int32).cpu().numpy().view(np.uint32)
+    out_d = dst.to(torch.int32).cpu().numpy().view(np.uint32)
+    del src, dst, deg
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(seed)
+    return out_s, out_d, _weights(rng, len(out_s), weights)
+
+
 def chung_lu(n, m, gamma=2.2, seed=1, weights="uniform"):
     rng = np.random.default_rng(seed)
     ew = np.power(np.arange(1, n + 1, dtype=np.float64), -1.0 / (gamma - 1.0))
