@@ -196,13 +196,15 @@ __device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, 
 
 // WPE: waves per SIMD the register budget is cut for (the LDS of one
 // workgroup already limits FP32 tiles to three per CU, BF16 tiles to four).
-template <typename T, int RM, bool WB, int WPE>
+// DP: the row pitch when known at compile time (128 for d = 128, so every
+// LDS offset is an immediate and the column loops unroll), 0: from a.dp.
+template <typename T, int RM, bool WB, int WPE, int DP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sgns_tile(Params a) {
     constexpr uint32_t GT = gt_pitch(sizeof(T), RM);  // pitch of the count / G^T tile (words)
     constexpr uint32_t EPV = 16 / sizeof(T);          // elements per 16-byte vector
     constexpr int NB = sizeof(T) == 4 ? 8 : 5;        // gathered vectors held per thread
     extern __shared__ __align__(16) unsigned char lds[];
-    const Lds L = lds_layout(a.dp, sizeof(T), RM);
+    const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM);
     TileDesc &td = *(TileDesc *)(lds + L.o_desc);
     const uint32_t *toks = (const uint32_t *)(lds + L.o_desc + offsetof(TileDesc, rin_tok));
     const uint32_t ldv = L.ldv;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t l16 = lane & 15, g4 = 4 * (lane >> 4);
     const uint32_t w = a.window, k = a.negative, B = a.bgroup;
-    const uint32_t dp = a.dp, d = a.d, dtiles = dp >> 4;
+    const uint32_t dp = DP ? (uint32_t)DP : a.dp, d = a.d, dtiles = dp >> 4;
     T *syn0 = (T *)a.syn0;
     T *syn1 = (T *)a.syn1;
     unsigned long long my_pairs = 0, my_tiles = 0;
@@ -617,28 +619,38 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     if (lane == 0) atomicAdd(&a.counters[C_PAIRS], my_pairs);
 }
 
-template <typename T, int RM, bool WB, int WPE>
-hipError_t occupancy(felog_ctx *c, int *per_cu) {
+template <typename T, int RM, bool WB, int WPE, int DP>
+hipError_t occupancy_dp(felog_ctx *c, int *per_cu) {
     const Lds L = lds_layout(c->dp, sizeof(T), RM);
-    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE>,
+    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE, DP>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE>, kBlock, L.total);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE, DP>, kBlock, L.total);
     if (*per_cu < 1) *per_cu = 1;
     return e;
 }
 
-// Persistent grid: the tile shape's workgroups per CU (at most what fits).
 template <typename T, int RM, bool WB, int WPE>
-hipError_t launch(felog_ctx *c, const Params &a) {
+hipError_t occupancy(felog_ctx *c, int *per_cu) {
+    return c->dp == 128 ? occupancy_dp<T, RM, WB, WPE, 128>(c, per_cu) : occupancy_dp<T, RM, WB, WPE, 0>(c, per_cu);
+}
+
+// Persistent grid: the tile shape's workgroups per CU (at most what fits).
+template <typename T, int RM, bool WB, int WPE, int DP>
+hipError_t launch_dp(felog_ctx *c, const Params &a) {
     int per_cu = 0;
-    hipError_t e = occupancy<T, RM, WB, WPE>(c, &per_cu);
+    hipError_t e = occupancy_dp<T, RM, WB, WPE, DP>(c, &per_cu);
     if (e != hipSuccess) return e;
     if (c->tile_gpc && (int)c->tile_gpc < per_cu) per_cu = (int)c->tile_gpc;
     const unsigned blocks = (unsigned)per_cu * c->cu_count;
     const Lds L = lds_layout(c->dp, sizeof(T), RM);
-    k_sgns_tile<T, RM, WB, WPE><<<blocks, kBlock, L.total, c->stream>>>(a);
+    k_sgns_tile<T, RM, WB, WPE, DP><<<blocks, kBlock, L.total, c->stream>>>(a);
     return hipGetLastError();
+}
+
+template <typename T, int RM, bool WB, int WPE>
+hipError_t launch(felog_ctx *c, const Params &a) {
+    return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128>(c, a) : launch_dp<T, RM, WB, WPE, 0>(c, a);
 }
 
 template <typename T> constexpr int kWpeLow = sizeof(T) == 4 ? 2 : 3;
