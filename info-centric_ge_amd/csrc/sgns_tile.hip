@@ -27,6 +27,7 @@
 // LDS footprint allows three at d = 128 in f32).
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <type_traits>
 #include "felog_dev.h"
 #include "felog_internal.h"
 #include "sgns_common.h"
@@ -54,25 +55,32 @@ __host__ __device__ constexpr uint32_t gt_pitch(uint32_t esize, uint32_t rmax) {
     return esize == 4 ? rmax + 4 : (rmax + 31) / 32 * 32 + 4;
 }
 
-// Groups of one tile with negatives (k >= 1): each has a centre and k
-// output slots, so at most OMAX / 2.
-constexpr uint32_t WROWS = OMAX / 2;
+// Groups of one tile: each has at least one centre and k output slots of
+// negatives, and the first group always fits, so at most
+// max(1, OMAX / (k + 1)) (5 at k = 5).
+__host__ __device__ inline uint32_t tile_wrows(uint32_t k) {
+    const uint32_t g = OMAX / (k + 1);
+    return g ? g : 1;
+}
 
 // LDS: descriptor | RM input rows then OMAX output rows (pitch ldv) | the
-// OMAX-row count / G^T tile, then WROWS rows of per-group context counts
-// (same pitch) | per-output-row masks of negative slots.
+// OMAX-row count / G^T tile, then one row of context counts per group
+// (same pitch) | per-output-row masks of negative slots.  At k = 5 the
+// bf16 layout fits five workgroups in a CU's 160 KB.
 struct Lds {
-    uint32_t ldv;  // row pitch of staged rows, in elements
+    uint32_t ldv;    // row pitch of staged rows, in elements
+    uint32_t wrows;  // rows of per-group context counts
     size_t o_desc, o_rows, o_g, o_mask, total;
 };
 
-__host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax) {
+__host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax, uint32_t k) {
     Lds L;
     L.ldv = dp + 16 / esize;  // 16-byte skew between rows
+    L.wrows = tile_wrows(k);
     size_t o = 0;
     L.o_desc = o; o += kDescBytes;
     L.o_rows = o; o += (size_t)(rmax + OMAX) * L.ldv * esize;
-    L.o_g = o;    o += (size_t)(OMAX + WROWS) * gt_pitch(esize, rmax) * 4;
+    L.o_g = o;    o += (size_t)(OMAX + L.wrows) * gt_pitch(esize, rmax) * 4;
     L.o_mask = o; o += (size_t)OMAX * 4;
     L.total = o;
     return L;
@@ -198,13 +206,15 @@ __device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, 
 // workgroup already limits FP32 tiles to three per CU, BF16 tiles to four).
 // DP: the row pitch when known at compile time (128 for d = 128, so every
 // LDS offset is an immediate and the column loops unroll), 0: from a.dp.
-template <typename T, int RM, bool WB, int WPE, int DP>
+// DBG: the test hook (products diverted to the debug buffers, tables left
+// alone); the production kernels carry none of its code.
+template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sgns_tile(Params a) {
     constexpr uint32_t GT = gt_pitch(sizeof(T), RM);  // pitch of the count / G^T tile (words)
     constexpr uint32_t EPV = 16 / sizeof(T);          // elements per 16-byte vector
     constexpr int NB = sizeof(T) == 4 ? 8 : 5;        // gathered vectors held per thread
     extern __shared__ __align__(16) unsigned char lds[];
-    const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM);
+    const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM, a.negative);
     TileDesc &td = *(TileDesc *)(lds + L.o_desc);
     const uint32_t *toks = (const uint32_t *)(lds + L.o_desc + offsetof(TileDesc, rin_tok));
     const uint32_t ldv = L.ldv;
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // rounded float quotient is exact)
     const float inv_k = k ? 1.f / (float)k : 0.f;
     // f32 write-back tiles start from the staged rows (acc = old + delta)
-    const bool acc_old = WB && sizeof(T) == 4 && a.dbg_max == 0;
+    constexpr bool acc_old = WB && sizeof(T) == 4 && !DBG;
     // stochastic-rounding stream of this thread for the launch (bf16 tables)
     uint32_t rs = 0;
     if constexpr (sizeof(T) == 2) {
@@ -280,7 +290,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 
     for (uint64_t nxt; tile < n_tiles; tile = nxt) {
         if (tid < (uint32_t)kDescVec) ((uint4 *)&td)[tid] = pre;
-        for (uint32_t i = tid; i < ((OMAX + WROWS) * GT + OMAX) / 4; i += kBlock)
+        for (uint32_t i = tid; i < ((OMAX + L.wrows) * GT + OMAX) / 4; i += kBlock)
             ((uint4 *)cnt)[i] = make_uint4(0, 0, 0, 0);  // counts, W and the masks
         __syncthreads();
         nxt = next_live();
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
         __syncthreads();
 
-        const bool dbg_on = a.dbg_max && tile < a.dbg_max;
+        const bool dbg_on = DBG && tile < a.dbg_max;
         if (dbg_on) {
             if (tid == 0) {
                 uint32_t *gq = a.dbg_groups + tile * 5;
@@ -479,7 +489,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     f4 acc = {0.f, 0.f, 0.f, 0.f};
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, ghi, acc, 0, 0, 0);
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, glo, acc, 0, 0, 0);
-                    if (a.dbg_max) {
+                    if constexpr (DBG) {
                         if (dbg_on)
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
@@ -516,7 +526,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, ghi[kb], acc, 0, 0, 0);
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, glo[kb], acc, 0, 0, 0);
                     }
-                    if (a.dbg_max) {
+                    if constexpr (DBG) {
                         if (dbg_on)
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
@@ -531,85 +541,86 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
 
         // ---- 5/6 on f32 MFMA (16x16x4), operands swapped as above -------------
-        // dVin = G * Uout  (R16 x dp, K = O16)
-        for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
-            const uint32_t r = rt * 16 + l16;
-            const LaneRow<T> row = lane_row(syn0, td.rin_tok, r, n_in, dp);
-            // G[r'][k], k = 16*kb' + 4*(lane/16) + i: one column of G^T per k,
-            // the same for every column block of this row block
-            const bool o2 = O16 > 16;
-            float gx[8];
-            {
+        // dVin = G * Uout  (R16 x dp, K = O16: KQ blocks of 16 output rows)
+        auto dv_rows = [&](auto kq) {
+            constexpr int KQ = decltype(kq)::value;
+            for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
+                const uint32_t r = rt * 16 + l16;
+                const LaneRow<T> row = lane_row(syn0, td.rin_tok, r, n_in, dp);
+                // G[r'][k], k = 16*q + 4*(lane/16) + i: one column of G^T per
+                // k, the same for every column block of this row block
+                float gx[4 * KQ];
                 const float *ga = Gt + g4 * GT + rt * 16 + l16;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) gx[i] = ga[i * GT];
+                for (int q = 0; q < KQ; ++q)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) gx[4 + i] = o2 ? ga[(16 + i) * GT] : 0.f;
-            }
-            for (uint32_t dt = wave; dt < dtiles; dt += 4) {
-                const uint32_t col0 = dt * 16 + g4;
-                const T *ub = Us + g4 * ldv + dt * 16 + l16;
-                float ux[8];
+                    for (int i = 0; i < 4; ++i) gx[4 * q + i] = ga[(16 * q + i) * GT];
+                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                    const uint32_t col0 = dt * 16 + g4;
+                    const T *ub = Us + g4 * ldv + dt * 16 + l16;
+                    float ux[4 * KQ];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) ux[i] = load_elem(ub + i * ldv);
-                if (o2)
+                    for (int q = 0; q < KQ; ++q)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) ux[4 + i] = load_elem(ub + (16 + i) * ldv);
-                f4 acc = acc_old ? *(const f4 *)(Vs + r * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
+                        for (int i = 0; i < 4; ++i) ux[4 * q + i] = load_elem(ub + (16 * q + i) * ldv);
+                    f4 acc = acc_old ? *(const f4 *)(Vs + r * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ux[i], gx[i], acc, 0, 0, 0);
-                if (o2)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ux[4 + i], gx[4 + i], acc, 0, 0, 0);
-                if (a.dbg_max) {
-                    if (dbg_on)
-                        for (int j = 0; j < 4; ++j)
-                            if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
-                } else {
-                    store_row<T, WB>(row, Vs + r * ldv, col0, acc, rs);
+                    for (int i = 0; i < 4 * KQ; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ux[i], gx[i], acc, 0, 0, 0);
+                    if constexpr (DBG) {
+                        if (dbg_on)
+                            for (int j = 0; j < 4; ++j)
+                                if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
+                    } else {
+                        store_row<T, WB>(row, Vs + r * ldv, col0, acc, rs);
+                    }
                 }
             }
-        }
-        // dUout = G^T * Vin  (O16 x dp, K = R16)
-        for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
-            const uint32_t o = ot * 16 + l16;
-            const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
-            // G^T[o][k]: four consecutive words per 16-deep block, the same
-            // for every column block of this row block
-            constexpr int KB = RM / 16;
-            const uint32_t nkb = R16 >> 4;
-            f4 gx[KB];
-            {
+        };
+        if (O16 > 16) dv_rows(std::integral_constant<int, 2>{});
+        else dv_rows(std::integral_constant<int, 1>{});
+        // dUout = G^T * Vin  (O16 x dp, K = R16: KQ blocks of 16 input rows)
+        auto du_rows = [&](auto kq) {
+            constexpr int KQ = decltype(kq)::value;
+            for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
+                const uint32_t o = ot * 16 + l16;
+                const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
+                // G^T[o][k]: four consecutive words per 16-deep block, the
+                // same for every column block of this row block
+                f4 gx[KQ];
                 const float *ga = Gt + o * GT + g4;
 #pragma unroll
-                for (int q = 0; q < KB; ++q)
-                    gx[q] = (uint32_t)q < nkb ? *(const f4 *)(ga + 16 * q) : f4{0.f, 0.f, 0.f, 0.f};
-            }
-            for (uint32_t dt = wave; dt < dtiles; dt += 4) {
-                const uint32_t col0 = dt * 16 + g4;
-                const T *vb = Vs + g4 * ldv + dt * 16 + l16;
-                float vx[4 * KB];
+                for (int q = 0; q < KQ; ++q) gx[q] = *(const f4 *)(ga + 16 * q);
+                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                    const uint32_t col0 = dt * 16 + g4;
+                    const T *vb = Vs + g4 * ldv + dt * 16 + l16;
+                    float vx[4 * KQ];
 #pragma unroll
-                for (int q = 0; q < KB; ++q)
-                    if ((uint32_t)q < nkb)
+                    for (int q = 0; q < KQ; ++q)
 #pragma unroll
                         for (int i = 0; i < 4; ++i) vx[4 * q + i] = load_elem(vb + (16 * q + i) * ldv);
-                f4 acc = acc_old ? *(const f4 *)(Us + o * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
+                    f4 acc = acc_old ? *(const f4 *)(Us + o * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int q = 0; q < KB; ++q)
-                    if ((uint32_t)q < nkb)
+                    for (int q = 0; q < KQ; ++q)
 #pragma unroll
                         for (int i = 0; i < 4; ++i)
                             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(vx[4 * q + i], gx[q][i], acc, 0, 0, 0);
-                if (a.dbg_max) {
-                    if (dbg_on)
-                        for (int j = 0; j < 4; ++j)
-                            if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
-                } else {
-                    store_row<T, WB>(row, Us + o * ldv, col0, acc, rs);
+                    if constexpr (DBG) {
+                        if (dbg_on)
+                            for (int j = 0; j < 4; ++j)
+                                if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
+                    } else {
+                        store_row<T, WB>(row, Us + o * ldv, col0, acc, rs);
+                    }
                 }
             }
+        };
+        switch (R16 >> 4) {
+            case 1: du_rows(std::integral_constant<int, 1>{}); break;
+            case 2: du_rows(std::integral_constant<int, 2>{}); break;
+            case 3: du_rows(std::integral_constant<int, 3>{}); break;
+            default:
+                if constexpr (RM >= 64) du_rows(std::integral_constant<int, 4>{});
+                break;
         }
         ++my_tiles;
         __syncthreads();  // the next tile reuses the LDS
@@ -619,13 +630,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     if (lane == 0) atomicAdd(&a.counters[C_PAIRS], my_pairs);
 }
 
-template <typename T, int RM, bool WB, int WPE, int DP>
+template <typename T> constexpr int kWpeLow = sizeof(T) == 4 ? 2 : 3;
+
+template <typename T, int RM, bool WB, int WPE, int DP, bool DBG = false>
 hipError_t occupancy_dp(felog_ctx *c, int *per_cu) {
-    const Lds L = lds_layout(c->dp, sizeof(T), RM);
-    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE, DP>,
+    const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative);
+    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE, DP, DBG>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE, DP>, kBlock, L.total);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE, DP, DBG>, kBlock,
+                                                     L.total);
     if (*per_cu < 1) *per_cu = 1;
     return e;
 }
@@ -636,24 +650,25 @@ hipError_t occupancy(felog_ctx *c, int *per_cu) {
 }
 
 // Persistent grid: the tile shape's workgroups per CU (at most what fits).
-template <typename T, int RM, bool WB, int WPE, int DP>
+template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
 hipError_t launch_dp(felog_ctx *c, const Params &a) {
     int per_cu = 0;
-    hipError_t e = occupancy_dp<T, RM, WB, WPE, DP>(c, &per_cu);
+    hipError_t e = occupancy_dp<T, RM, WB, WPE, DP, DBG>(c, &per_cu);
     if (e != hipSuccess) return e;
     if (c->tile_gpc && (int)c->tile_gpc < per_cu) per_cu = (int)c->tile_gpc;
     const unsigned blocks = (unsigned)per_cu * c->cu_count;
-    const Lds L = lds_layout(c->dp, sizeof(T), RM);
-    k_sgns_tile<T, RM, WB, WPE, DP><<<blocks, kBlock, L.total, c->stream>>>(a);
+    const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative);
+    k_sgns_tile<T, RM, WB, WPE, DP, DBG><<<blocks, kBlock, L.total, c->stream>>>(a);
     return hipGetLastError();
 }
 
 template <typename T, int RM, bool WB, int WPE>
 hipError_t launch(felog_ctx *c, const Params &a) {
-    return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128>(c, a) : launch_dp<T, RM, WB, WPE, 0>(c, a);
+    if (a.dbg_max)  // the test hook: one register budget per table type
+        return c->dp == 128 ? launch_dp<T, RM, WB, kWpeLow<T> + 1, 128, true>(c, a)
+                            : launch_dp<T, RM, WB, kWpeLow<T> + 1, 0, true>(c, a);
+    return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128, false>(c, a) : launch_dp<T, RM, WB, WPE, 0, false>(c, a);
 }
-
-template <typename T> constexpr int kWpeLow = sizeof(T) == 4 ? 2 : 3;
 
 template <typename T, int RM, bool WB>
 hipError_t launch_wpe(felog_ctx *c, const Params &a) {
