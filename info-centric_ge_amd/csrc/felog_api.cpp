@@ -66,7 +66,7 @@ void drop_round_graph(felog_ctx *c) {
 namespace {
 
 void free_all(felog_ctx *c) {
-    void *ptrs[] = {c->d_esrc, c->d_edst, c->d_ew, c->d_row_ptr, c->d_col, c->d_w, c->d_vrec, c->d_erec,
+    void *ptrs[] = {c->d_esrc, c->d_edst, c->d_ew, c->d_row_ptr, c->d_col, c->d_w, c->d_vrec, c->d_vwalk, c->d_erec,
                     c->d_deg, c->d_vocab_order, c->d_vocab_rank, c->d_in_vocab, c->d_uni,
                     c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
                     c->d_scores, c->d_active, c->d_sel, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,

@@ -11,10 +11,20 @@ namespace felog {
 
 // Device record of one CSR vertex: first CSR entry and degree.
 struct VertexRec { uint32_t begin, deg; };
+// In a walk's vertex records the degree carries this bit when the degree is
+// 2 and the first neighbour is the vertex itself (such a vertex ends a walk,
+// include/walk.hpp:977-979).
+constexpr uint32_t kSelfFirst = 0x80000000u;
 // Device record of one CSR entry j of vertex v: the alias threshold of
-// column j, the neighbour in column j, the neighbour in column alias(j) and
-// alias(j) itself.  One 16-byte gather decides a walk step.
-struct EdgeRec { uint32_t thresh, col, alias_col, alias_idx; };
+// column j, the neighbour in column j, the neighbour in column alias(j),
+// alias(j) itself, and both neighbours' walk vertex records.  One 32-byte
+// gather (one cache line) decides a walk step and hands the next step its
+// vertex.
+struct EdgeRec {
+    uint32_t thresh, col, alias_col, alias_idx;
+    VertexRec col_vr, alias_vr;
+};
+static_assert(sizeof(EdgeRec) == 32, "one 32-byte gather per walk step");
 
 struct Corpus {
     uint32_t round = 0;
@@ -81,6 +91,7 @@ struct felog_ctx {
     uint32_t *d_col = nullptr;           // E
     float *d_w = nullptr;                // E
     felog::VertexRec *d_vrec = nullptr;  // V
+    felog::VertexRec *d_vwalk = nullptr; // V: vrec with the kSelfFirst bit
     felog::EdgeRec *d_erec = nullptr;    // E
     uint32_t *d_deg = nullptr;           // V
 

@@ -77,6 +77,29 @@ __global__ void k_vrec(const uint64_t *__restrict__ row_ptr, uint32_t V, VertexR
     }
 }
 
+// Walk vertex records: vrec with kSelfFirst on degree-2 vertices whose
+// first neighbour is themselves.
+__global__ void k_vwalk(const VertexRec *__restrict__ vrec, const uint32_t *__restrict__ col, uint32_t V,
+                        VertexRec *__restrict__ vwalk) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        VertexRec r = vrec[v];
+        if (r.deg == 2 && col[r.begin] == v) r.deg |= kSelfFirst;
+        vwalk[v] = r;
+    }
+}
+
+// Both neighbours' walk records into every edge record.
+__global__ void k_edge_vr(const VertexRec *__restrict__ vwalk, uint64_t E, EdgeRec *__restrict__ erec) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
+        EdgeRec r = erec[e];
+        r.col_vr = vwalk[r.col];
+        r.alias_vr = vwalk[r.alias_col];
+        erec[e] = r;
+    }
+}
+
 // Threshold of an alias column: P(keep column) = thresh / 2^32.
 __device__ __forceinline__ uint32_t alias_quantise(double p) {
     if (p >= 1.0) return 0xFFFFFFFFu;
@@ -120,7 +143,7 @@ __global__ void k_alias(const uint32_t *__restrict__ order, uint32_t V,
             const uint32_t l = stk[b + n - nl];
             --nl;
             const double ps = p[b + s];
-            erec[b + s] = EdgeRec{alias_quantise(ps), col[b + s], col[b + l], l};
+            erec[b + s] = EdgeRec{alias_quantise(ps), col[b + s], col[b + l], l, {}, {}};
             const double pl = (p[b + l] + ps) - 1.0;
             p[b + l] = pl;
             if (pl < 1.0) stk[b + ns++] = l;
@@ -129,11 +152,11 @@ __global__ void k_alias(const uint32_t *__restrict__ order, uint32_t V,
         while (nl > 0) {
             const uint32_t l = stk[b + n - nl];
             --nl;
-            erec[b + l] = EdgeRec{0xFFFFFFFFu, col[b + l], col[b + l], l};
+            erec[b + l] = EdgeRec{0xFFFFFFFFu, col[b + l], col[b + l], l, {}, {}};
         }
         while (ns > 0) {
             const uint32_t s = stk[b + --ns];
-            erec[b + s] = EdgeRec{0xFFFFFFFFu, col[b + s], col[b + s], s};
+            erec[b + s] = EdgeRec{0xFFFFFFFFu, col[b + s], col[b + s], s, {}, {}};
         }
     }
 }
@@ -314,6 +337,13 @@ hipError_t build_alias(felog_ctx *c) {
     k_alias<<<grid_for(c->V, 64, 1u << 20), 64, 0, c->stream>>>(c->d_vocab_order, c->V, c->d_row_ptr,
                                                                 c->d_w, c->d_col, p, stk, c->d_erec);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMalloc(&c->d_vwalk, (uint64_t)c->V * sizeof(VertexRec) + 8));
+    k_vwalk<<<grid_for(c->V, 256, 65535), 256, 0, c->stream>>>(c->d_vrec, c->d_col, c->V, c->d_vwalk);
+    HIP_TRY(hipGetLastError());
+    if (E) {
+        k_edge_vr<<<grid_for(E, 256, 1u << 16), 256, 0, c->stream>>>(c->d_vwalk, E, c->d_erec);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     dfree(p);
     dfree(stk);

@@ -29,7 +29,7 @@ namespace felog {
 namespace {
 
 struct WalkArgs {
-    const VertexRec *__restrict__ vrec;
+    const VertexRec *__restrict__ vwalk;  // start vertices' walk records
     const EdgeRec *__restrict__ erec;
     const uint32_t *__restrict__ active;
     const RoundState *rs;  // round and walkers started (device)
@@ -65,6 +65,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
     bool active = false;
     bool more = true;  // wave-uniform: the work cursor may still hold walkers
     uint32_t w = 0, v = 0, s = 0, attempt = 0, lim = 0;
+    VertexRec cur{0, 0};  // walk record of v (degree with kSelfFirst)
     uint64_t ob = kNoOverflow;
     WalkStats st;
     walk_stats_init(st);
@@ -86,6 +87,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                 if (idx < n_active) {
                     w = a.active[idx];
                     v = w % a.V;
+                    cur = a.vwalk[v];
                     lim = a.fixed_len ? a.fixed_len[v] : 0u;
                     s = 0;
                     attempt = 0;
@@ -103,17 +105,16 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
         if (s + 1 >= a.max_len || (lim && s + 1 >= lim)) {
             stop = true;
         } else {
-            const VertexRec vr = a.vrec[v];
-            if (vr.deg == 0) {
-                stop = true;
-            } else if (vr.deg == 2 && a.erec[vr.begin].col == v) {
-                stop = true;
+            const uint32_t deg = cur.deg & ~kSelfFirst;
+            if (deg == 0 || (cur.deg & kSelfFirst)) {
+                stop = true;  // degree 0, or degree 2 with itself first
             } else {
                 const u32x4 r = draw(a.seed, s, DOM_WALK | attempt, w, round);
-                const uint32_t j = alias_bucket(r.x, vr.deg);
-                const EdgeRec e = a.erec[vr.begin + j];
+                const uint32_t j = alias_bucket(r.x, deg);
+                const EdgeRec e = a.erec[cur.begin + j];
                 // alias step, or the reference's uniform neighbour (cn_data)
-                const uint32_t x = (a.cn_data || r.y < e.thresh) ? e.col : e.alias_col;
+                const bool first = a.cn_data || r.y < e.thresh;
+                const uint32_t x = first ? e.col : e.alias_col;
                 if (s == 0 && x == v) {
                     if (++attempt >= kMaxStartAttempts) stop = true;
                 } else {
@@ -148,13 +149,15 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                     };
                     bool trace = false;
                     if (a.cn_data)
-                        trace = cn_trace(vr.deg, a.vrec[x].deg, a.cn_data[vr.begin + j], uniform53(r.y, r.z));
+                        trace = cn_trace(deg, e.col_vr.deg & ~kSelfFirst, a.cn_data[cur.begin + j],
+                                         uniform53(r.y, r.z));
                     stop = append(x);
                     if (trace) {
                         // include/walk.hpp:1063-1094: back to v, which stays current
                         if (!stop) stop = s + 1 >= a.max_len || append(v);
                     } else {
                         v = x;
+                        cur = first ? e.col_vr : e.alias_vr;
                     }
                 }
             }
@@ -341,7 +344,7 @@ hipError_t walk_enqueue(felog_ctx *c) {
     k_select<<<grid_for(W, 256, 65535), 256, 0, c->stream>>>(c->d_sel, W, c->d_active, c->d_rs);
     HIP_TRY(hipGetLastError());
     WalkArgs a;
-    a.vrec = c->d_vrec;
+    a.vwalk = c->d_vwalk;
     a.erec = c->d_erec;
     a.active = c->d_active;
     a.rs = c->d_rs;
