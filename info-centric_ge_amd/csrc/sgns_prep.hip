@@ -122,10 +122,13 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             {
                 const bool valid = lane < npos;
                 const uint32_t t = valid ? stok[lo + lane] : 0xFFFFFFFFu;
+                // first lane holding the same row: a uniform loop of lane
+                // broadcasts (no memory round trip per comparison)
                 uint32_t first = lane;
-                if (valid)
-                    for (uint32_t q = 0; q < lane; ++q)
-                        if (stok[lo + q] == t) { first = q; break; }
+                for (uint32_t q = 0; q < npos; ++q) {
+                    const uint32_t tq = __builtin_amdgcn_readlane(t, q);
+                    if (q < first && tq == t) first = q;
+                }
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
@@ -146,11 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 uint32_t t = 0xFFFFFFFFu;
                 if (valid) t = lane < nc ? stok[c0 + lane] : sneg[lane - nc];
                 uint32_t first = lane;
-                if (valid)
-                    for (uint32_t q = 0; q < lane; ++q) {
-                        const uint32_t tq = q < nc ? stok[c0 + q] : sneg[q - nc];
-                        if (tq == t) { first = q; break; }
-                    }
+                for (uint32_t q = 0; q < nl; ++q) {
+                    const uint32_t tq = __builtin_amdgcn_readlane(t, q);
+                    if (q < first && tq == t) first = q;
+                }
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
