@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end plain vs BF16 + budget runs")
     ap.add_argument("--e2e-no-auc", action="store_true", help="end-to-end runs on the whole graph, no AUC")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tile shape (profiling runs: one tile kernel variant)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -248,7 +250,7 @@ def main():
     # restored after each pass) and installs the fastest
     default_shape = f.tile_shape()
     t0 = time.perf_counter()
-    tuned = f.tune_sgns(sample=0, reps=2)
+    tuned = [] if args.no_tune else f.tune_sgns(sample=0, reps=2)
     tune_s = time.perf_counter() - t0
     chosen = f.tile_shape()
     print(f"[bench] tile autotune {len(tuned)} shapes in {tune_s:.1f}s: chosen {chosen}", file=sys.stderr,
