@@ -202,11 +202,16 @@ felog_status felog_run_graph(felog_ctx *ctx, uint32_t max_rounds);
  * reference's HogBatch sizes its MKL GEMMs per sentence pair,
  * include/dsgl.hpp:1099-1159).  Zero fields take the defaults. */
 typedef struct felog_tile_shape {
-    uint32_t rows;            /* input positions (windows) of a tile: 48 or 64;
-                                 48 needs batch centres + 2 window <= 48      */
+    uint32_t rows;            /* input rows of the tile kernel: 48 or 64 (48
+                                 needs batch centres + 2 window <= 48), or 32:
+                                 tiles of at most 32 distinct input rows run on
+                                 a 32-row kernel (smaller LDS footprint, more
+                                 workgroups per CU), the rest on the 48/64-row
+                                 one (the default)                            */
     uint32_t waves_per_simd;  /* register budget: FP32 2 or 3, BF16 3 or 4  */
     uint32_t groups_per_cu;   /* persistent workgroups per CU (at most what
-                                 LDS and registers allow)                   */
+                                 LDS and registers allow; with rows 32, of
+                                 the 32-row kernel)                         */
     float tile_ms;            /* felog_tune_sgns: tile-kernel time          */
 } felog_tile_shape;
 felog_status felog_tile_shape_set(felog_ctx *ctx, const felog_tile_shape *shape);

@@ -792,7 +792,8 @@ uint32_t group_centres(const felog_ctx *c) {
 }
 
 felog_status check_shape(felog_ctx *c, const felog_tile_shape &s) {
-    if (s.rows != 0 && s.rows != 48 && s.rows != 64) return fail(c, FELOG_ERR_ARG, "tile rows must be 48 or 64");
+    if (s.rows != 0 && s.rows != 32 && s.rows != 48 && s.rows != 64)
+        return fail(c, FELOG_ERR_ARG, "tile rows must be 32, 48 or 64");
     if (s.rows == 48 && group_centres(c) + 2 * c->opt.window > 48)
         return fail(c, FELOG_ERR_ARG, "48-row tiles need batch centres + 2 window <= 48");
     const uint32_t lo = tile_wpe_low(c);
@@ -827,7 +828,7 @@ felog_status felog_tile_shape_set(felog_ctx *c, const felog_tile_shape *shape) {
 felog_status felog_tile_shape_get(felog_ctx *c, felog_tile_shape *out) {
     if (!c || !out) return FELOG_ERR_ARG;
     TRY(c, hipSetDevice(c->device));
-    out->rows = c->tile_rows ? c->tile_rows : sgns_tile_rows(group_centres(c), c->opt.window);
+    out->rows = c->tile_rows ? c->tile_rows : 32u;  // default: the 32-row split
     out->waves_per_simd = c->tile_wpe ? c->tile_wpe : tile_wpe_low(c) + 1;
     int per_cu = 0;
     TRY(c, tile_occupancy(c, out->rows, out->waves_per_simd, &per_cu));
@@ -851,7 +852,7 @@ felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t r
     // every shape: rows x register budget x workgroups per CU up to what fits
     std::vector<felog_tile_shape> cand;
     const uint32_t lo = tile_wpe_low(c);
-    for (uint32_t rows : {48u, 64u}) {
+    for (uint32_t rows : {32u, 48u, 64u}) {
         if (rows == 48 && group_centres(c) + 2 * c->opt.window > 48) continue;
         for (uint32_t wpe : {lo, lo + 1}) {
             int per_cu = 0;

@@ -236,7 +236,8 @@ hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_t
     a.window = c->opt.window;
     a.negative = c->opt.negative;
     a.bgroup = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
-    a.rmax = c->tile_rows ? c->tile_rows : sgns_tile_rows(a.bgroup, c->opt.window);
+    // tiles are packed for 48 or 64 rows (a 32-row shape splits them by size)
+    a.rmax = c->tile_rows >= 48 ? c->tile_rows : sgns_tile_rows(a.bgroup, c->opt.window);
     a.compat_break = c->opt.compat_vocab_break;
     a.writeback = c->opt.update_mode != FELOG_UPDATE_ATOMIC;
     a.seed = c->opt.seed;

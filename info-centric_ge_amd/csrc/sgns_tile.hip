@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // stochastic-rounding stream of this thread for the launch (bf16 tables)
     uint32_t rs = 0;
     if constexpr (sizeof(T) == 2) {
-        rs = draw(a.seed, blockIdx.x * kBlock + tid, DOM_ROUND_BF16, 0u, round).x;
+        rs = draw(a.seed, a.sr_base + blockIdx.x * kBlock + tid, DOM_ROUND_BF16, 0u, round).x;
         if (rs == 0) rs = 0x9E3779B9u;
     }
 
@@ -273,7 +273,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     uint64_t win = blockIdx.x, live = 0;
     auto scan = [&]() {
         const uint64_t t = win + (uint64_t)lane * gridDim.x;
-        live = __ballot(t < n_tiles && desc_words[t * (kDescBytes / 4) + 2] != 0u);
+        bool ok = false;
+        if (t < n_tiles) {
+            const uint32_t *dw = desc_words + t * (kDescBytes / 4);
+            const uint32_t nin = dw[6];
+            ok = dw[2] != 0u && nin >= a.nin_lo && nin <= a.nin_hi;  // nc, n_in
+        }
+        live = __ballot(ok);
     };
     auto next_live = [&]() -> uint64_t {
         while (live == 0) {
@@ -652,41 +658,71 @@ hipError_t occupancy(felog_ctx *c, int *per_cu) {
 }
 
 // Persistent grid: the tile shape's workgroups per CU (at most what fits).
+// gpc: workgroups per CU (0: as many as fit); *blocks receives the grid.
 template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
-hipError_t launch_dp(felog_ctx *c, const Params &a) {
+hipError_t launch_dp(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
     int per_cu = 0;
     hipError_t e = occupancy_dp<T, RM, WB, WPE, DP, DBG>(c, &per_cu);
     if (e != hipSuccess) return e;
-    if (c->tile_gpc && (int)c->tile_gpc < per_cu) per_cu = (int)c->tile_gpc;
-    const unsigned blocks = (unsigned)per_cu * c->cu_count;
+    if (gpc && (int)gpc < per_cu) per_cu = (int)gpc;
+    *blocks = (unsigned)per_cu * c->cu_count;
     const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative);
-    k_sgns_tile<T, RM, WB, WPE, DP, DBG><<<blocks, kBlock, L.total, c->stream>>>(a);
+    k_sgns_tile<T, RM, WB, WPE, DP, DBG><<<*blocks, kBlock, L.total, c->stream>>>(a);
     return hipGetLastError();
 }
 
 template <typename T, int RM, bool WB, int WPE>
-hipError_t launch(felog_ctx *c, const Params &a) {
+hipError_t launch(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
     if (a.dbg_max)  // the test hook: one register budget per table type
-        return c->dp == 128 ? launch_dp<T, RM, WB, kWpeLow<T> + 1, 128, true>(c, a)
-                            : launch_dp<T, RM, WB, kWpeLow<T> + 1, 0, true>(c, a);
-    return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128, false>(c, a) : launch_dp<T, RM, WB, WPE, 0, false>(c, a);
+        return c->dp == 128 ? launch_dp<T, RM, WB, kWpeLow<T> + 1, 128, true>(c, a, gpc, blocks)
+                            : launch_dp<T, RM, WB, kWpeLow<T> + 1, 0, true>(c, a, gpc, blocks);
+    return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128, false>(c, a, gpc, blocks)
+                        : launch_dp<T, RM, WB, WPE, 0, false>(c, a, gpc, blocks);
 }
 
 template <typename T, int RM, bool WB>
-hipError_t launch_wpe(felog_ctx *c, const Params &a) {
+hipError_t launch_wpe(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
     constexpr int lo = kWpeLow<T>;
-    return c->tile_wpe == (uint32_t)lo ? launch<T, RM, WB, lo>(c, a) : launch<T, RM, WB, lo + 1>(c, a);
+    return c->tile_wpe == (uint32_t)lo ? launch<T, RM, WB, lo>(c, a, gpc, blocks)
+                                       : launch<T, RM, WB, lo + 1>(c, a, gpc, blocks);
 }
 
 template <typename T, int RM>
-hipError_t launch_rm(felog_ctx *c, const Params &a) {
-    return a.writeback ? launch_wpe<T, RM, true>(c, a) : launch_wpe<T, RM, false>(c, a);
+hipError_t launch_rm(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
+    return a.writeback ? launch_wpe<T, RM, true>(c, a, gpc, blocks) : launch_wpe<T, RM, false>(c, a, gpc, blocks);
+}
+
+// The 48/64-row kernel (a.rmax rows), and the 32-row one.
+template <typename T>
+hipError_t launch_big(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
+    return a.rmax == 48 ? launch_rm<T, 48>(c, a, gpc, blocks) : launch_rm<T, 64>(c, a, gpc, blocks);
+}
+
+template <typename T>
+hipError_t launch_split(felog_ctx *c, Params a) {
+    // tiles of at most 32 input rows on the 32-row kernel (a smaller LDS
+    // footprint, more workgroups per CU), the rest on the 48/64-row one
+    unsigned blocks = 0;
+    a.nin_lo = 0;
+    a.nin_hi = 32;
+    a.sr_base = 0;
+    hipError_t e = launch_rm<T, 32>(c, a, c->tile_gpc, &blocks);
+    if (e != hipSuccess) return e;
+    a.nin_lo = 33;
+    a.nin_hi = RMAX;
+    a.sr_base = blocks * kBlock;
+    return launch_big<T>(c, a, 0, &blocks);
 }
 
 template <typename T>
 hipError_t occupancy_t(felog_ctx *c, uint32_t rows, uint32_t wpe, int *per_cu) {
     constexpr int lo = kWpeLow<T>;
     const bool wb = c->opt.update_mode != FELOG_UPDATE_ATOMIC;
+    if (rows == 32) {
+        if (wpe == (uint32_t)lo)
+            return wb ? occupancy<T, 32, true, lo>(c, per_cu) : occupancy<T, 32, false, lo>(c, per_cu);
+        return wb ? occupancy<T, 32, true, lo + 1>(c, per_cu) : occupancy<T, 32, false, lo + 1>(c, per_cu);
+    }
     if (rows == 48) {
         if (wpe == (uint32_t)lo)
             return wb ? occupancy<T, 48, true, lo>(c, per_cu) : occupancy<T, 48, false, lo>(c, per_cu);
@@ -699,10 +735,17 @@ hipError_t occupancy_t(felog_ctx *c, uint32_t rows, uint32_t wpe, int *per_cu) {
 
 }  // namespace
 
-hipError_t launch_tile(felog_ctx *c, const Params &a) {
-    if (c->opt.precision == FELOG_BF16)
-        return a.rmax == 48 ? launch_rm<uint16_t, 48>(c, a) : launch_rm<uint16_t, 64>(c, a);
-    return a.rmax == 48 ? launch_rm<float, 48>(c, a) : launch_rm<float, 64>(c, a);
+hipError_t launch_tile(felog_ctx *c, const Params &a0) {
+    const bool bf = c->opt.precision == FELOG_BF16;
+    // default: the split (tiles of walks up to 32 tokens on the 32-row kernel)
+    if (c->tile_rows == 0 || c->tile_rows == 32)
+        return bf ? launch_split<uint16_t>(c, a0) : launch_split<float>(c, a0);
+    Params a = a0;
+    a.nin_lo = 0;
+    a.nin_hi = RMAX;
+    a.sr_base = 0;
+    unsigned blocks = 0;
+    return bf ? launch_big<uint16_t>(c, a, c->tile_gpc, &blocks) : launch_big<float>(c, a, c->tile_gpc, &blocks);
 }
 
 }  // namespace sgns

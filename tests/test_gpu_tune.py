@@ -36,7 +36,7 @@ def ctx(src, dst, w, n, **kw):
 def shapes(f, prec):
     lo = 2 if prec == felog.FP32 else 3
     out = []
-    for rows in (48, 64):
+    for rows in (32, 48, 64):
         for wpe in (lo, lo + 1):
             f.set_tile_shape(rows, wpe, 0)
             for g in range(1, f.tile_shape()["groups_per_cu"] + 1):
