@@ -183,6 +183,7 @@ enum : int {
     C_EVALNUM = 11,    // vertices evaluated and not converged
     C_EVALCONV = 12,   // vertices flagged this evaluation
     C_DEBUG = 13,      // debug group cursor
+    C_BIGTILES = 14,   // SGNS tiles with more than 32 input rows
     kNumCounters = 16
 };
 

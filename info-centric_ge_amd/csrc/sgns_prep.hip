@@ -134,6 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
                 const uint32_t my = __shfl(idx, (int)first);
                 const uint32_t n_in = (uint32_t)__popcll(bal);
+                if (lane == 0 && n_in > 32) atomicAdd(&a.counters[C_BIGTILES], 1ull);
                 if (isf) td->rin_tok[idx] = t;
                 if (lane >= n_in) td->rin_tok[lane] = 0xFFFFFFFFu;
                 td->pos_rin[lane] = valid ? (uint8_t)my : 0;
@@ -260,7 +261,7 @@ hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_t
 hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
                     const SgnsDebug *dbg, float *ms) {
     HIP_TRY(sgns_reserve(c, n_sent, n_tok));
-    HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_DEBUG - C_WORDS + 1), c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_BIGTILES - C_WORDS + 1), c->stream));
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     HIP_TRY(sgns_enqueue(c, d_off, d_tok, dbg, c->ev1));
     HIP_TRY(hipEventRecord(c->ev2, c->stream));

@@ -261,6 +261,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     if constexpr (sizeof(T) == 2)
         for (uint32_t i = tid; i < (RM + OMAX) * ldv * 2 / 16; i += kBlock) ((uint4 *)Vs)[i] = make_uint4(0, 0, 0, 0);
 
+    // the 48/64-row launch of a split finds nothing to do when the prep
+    // counted no tile over 32 input rows
+    if (a.nin_lo > 32 && a.counters[C_BIGTILES] == 0) return;
     uint64_t n_tiles = (uint64_t)a.round_state->n_active + a.counters[C_TILES];
     if (n_tiles > a.tiles_cap) n_tiles = a.tiles_cap;
     const uint4 *descs = (const uint4 *)a.tiles;

@@ -152,6 +152,38 @@ def test_sgns_groups_vs_fp64(er):
     print(f"worst relative error vs FP64: {worst:.3e}")
 
 
+@pytest.mark.parametrize("rows", [32, 48, 64])
+def test_sgns_long_sentences_vs_fp64(er, rows):
+    """Sentences of 30-120 random tokens make tiles of up to 41 distinct
+    input rows: with the default 32-row split those over 32 rows run on the
+    48-row kernel in a second launch.  Every tile shape matches FP64."""
+    f, _ = er
+    rng = np.random.default_rng(17)
+    lens = rng.integers(30, 121, 60)
+    off_s = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    tok_s = rng.integers(0, 10_000, int(off_s[-1])).astype(np.uint32)
+    s0, s1 = _random_tables(10_000, 128, 9)
+    f.set_embeddings(s0, s1)
+    f.set_tile_shape(rows, 0, 0)
+    try:
+        got = f.debug_sgns(off_s, tok_s, 2, 0.025, 4096)
+    finally:
+        f.set_tile_shape()
+    ref = orc.sgns_groups(_tables_for_oracle(f), off_s, tok_s, s0, s1, d=128, window=10, negative=5, batch=0,
+                          alpha=0.025, rnd=2, seed=SEED, mode=0)
+    assert len(got["groups"]) == ref["n_groups"] > 0
+    assert max(int(g[3]) for g in got["groups"]) > 32  # tiles for the second kernel
+    key = {(int(g[0]), int(g[1])): i for i, g in enumerate(ref["groups"])}
+    for i, g in enumerate(got["groups"]):
+        j = key[(int(g[0]), int(g[1]))]
+        assert np.array_equal(g, ref["groups"][j])
+        n_in, n_out = int(g[3]), int(g[4])
+        for name, shape in (("logits", (n_in, n_out)), ("dv", (n_in, 128)), ("du", (n_out, 128))):
+            x = got[name][i][: shape[0], : shape[1]].astype(np.float64)
+            y = ref[name][j][: shape[0], : shape[1]]
+            assert np.linalg.norm(x - y) <= 1e-5 * max(np.linalg.norm(y), 1e-30), (rows, name, i)
+
+
 def test_sgns_groups_bf16_vs_fp64():
     """bf16 tables: X on bf16 MFMA from the exact bf16 rows; the gradient
     products split G into bf16 hi + lo (relative residue below 2^-16), so the
