@@ -23,8 +23,10 @@
 //   4. dVin and dUout, each 16x16 output tile written straight to the table.
 // Every loop index is advanced incrementally (no integer division in the
 // tile loop) and each lane's table row addresses are formed once per 16-row
-// block.  Latency of the gathers is hidden by several workgroups per CU (the
-// LDS footprint allows three at d = 128 in f32).
+// block.  Latency of the gathers is hidden by several workgroups per CU (at
+// d = 128 in f32 the LDS allows four of the 32-row instance, three of the
+// 48-row one).  Tiles with at most 32 distinct input rows run on the 32-row
+// instance, the rest on a 48/64-row instance launched after it.
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <type_traits>
@@ -205,7 +207,7 @@ __device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, 
 }
 
 // WPE: waves per SIMD the register budget is cut for (the LDS of one
-// workgroup already limits FP32 tiles to three per CU, BF16 tiles to four).
+// workgroup limits the occupancy first at d = 128).
 // DP: the row pitch when known at compile time (128 for d = 128, so every
 // LDS offset is an immediate and the column loops unroll), 0: from a.dp.
 // DBG: the test hook (products diverted to the debug buffers, tables left
@@ -660,8 +662,8 @@ hipError_t occupancy(felog_ctx *c, int *per_cu) {
     return c->dp == 128 ? occupancy_dp<T, RM, WB, WPE, 128>(c, per_cu) : occupancy_dp<T, RM, WB, WPE, 0>(c, per_cu);
 }
 
-// Persistent grid: the tile shape's workgroups per CU (at most what fits).
-// gpc: workgroups per CU (0: as many as fit); *blocks receives the grid.
+// Persistent grid of gpc workgroups per CU (0 or more than fit: as many as
+// fit); *blocks receives the grid.
 template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
 hipError_t launch_dp(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
     int per_cu = 0;
