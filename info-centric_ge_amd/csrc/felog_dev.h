@@ -37,7 +37,7 @@ enum : uint32_t {
     DOM_WINDOW = 0x20000000u,     // x = kept position of the centre, z = sentence
     DOM_NEGATIVE = 0x30000000u,   // y |= negative slot, x = group, z = sentence
     DOM_EVAL = 0x40000000u,       // y |= draw index, x = vertex, z = ~0
-    DOM_ROUND_BF16 = 0x50000000u, // x = thread of the tile launch, z = 0 (seeds xorshift32)
+    DOM_ROUND_BF16 = 0x50000000u, // x = thread, z = tile slot (seeds xorshift32)
     DOM_MLP = 0x60000000u,        // x = sample slot of a budget-MLP step, w = step
 };
 

@@ -184,7 +184,9 @@ enum : int {
     C_EVALCONV = 12,   // vertices flagged this evaluation
     C_DEBUG = 13,      // debug group cursor
     C_BIGTILES = 14,   // SGNS tiles with more than 32 input rows
-    kNumCounters = 16
+    C_CURSOR = 15,     // SGNS tile cursor (zero between launches)
+    C_DONE = 16,       // SGNS tile workgroups finished (zero between launches)
+    kNumCounters = 18
 };
 
 // graph_kernels.hip

@@ -60,9 +60,8 @@ struct Params {
     unsigned char *tiles;
     uint64_t tiles_cap;
     // tiles of this launch: n_in in [nin_lo, nin_hi] (a 32-row kernel takes
-    // the small tiles, a 48/64-row one the rest); sr_base offsets the bf16
-    // rounding streams of a second launch
-    uint32_t nin_lo, nin_hi, sr_base;
+    // the small tiles, a 48/64-row one the rest)
+    uint32_t nin_lo, nin_hi;
     unsigned long long *counters;
     // test hook: when dbg_max > 0 the tables are not touched
     uint32_t dbg_max;

@@ -15,18 +15,20 @@
 // taken before the tile's writes, i.e. HogBatch's corrM / dOut / dIn
 // (include/dsgl.hpp:1241-1310) over deduplicated rows.
 //
-// Work per tile and workgroup:
-//   1. descriptor into LDS, count tile cleared;
-//   2. row gathers issued into registers, pair counts built with LDS
-//      atomics while they are in flight, gathered rows stored to LDS;
-//   3. X and G (G^T overwrites the counts in place);
-//   4. dVin and dUout, each 16x16 output tile written straight to the table.
+// Work per tile and workgroup (its rows were gathered into registers while
+// the previous tile trained):
+//   1. count tile cleared, pair counts built with LDS atomics;
+//   2. gathered rows stored to LDS, the next descriptor to its slot;
+//   3. the next tile's row gathers issued into registers;
+//   4. X and G (G^T overwrites the counts in place);
+//   5. dVin and dUout, each 16x16 output tile written straight to the table.
 // Every loop index is advanced incrementally (no integer division in the
 // tile loop) and each lane's table row addresses are formed once per 16-row
-// block.  Latency of the gathers is hidden by several workgroups per CU (at
-// d = 128 in f32 the LDS allows four of the 32-row instance, three of the
-// 48-row one).  Tiles with at most 32 distinct input rows run on the 32-row
-// instance, the rest on a 48/64-row instance launched after it.
+// block.  Latency of the gathers is hidden by the pipeline and by several
+// workgroups per CU (at d = 128 in f32 the LDS allows four of the 32-row
+// instance, three of the 48-row one).  Tiles with at most 32 distinct input
+// rows run on the 32-row instance, the rest on a 48/64-row instance
+// launched after it.
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <type_traits>
@@ -65,14 +67,16 @@ __host__ __device__ inline uint32_t tile_wrows(uint32_t k) {
     return g ? g : 1;
 }
 
-// LDS: descriptor | RM input rows then OMAX output rows (pitch ldv) | the
-// OMAX-row count / G^T tile, then one row of context counts per group
-// (same pitch) | per-output-row masks of negative slots.  At k = 5 the
-// bf16 layout fits five workgroups in a CU's 160 KB.
+// LDS: two descriptor slots (this tile's, the next one's) | RM input rows
+// then OMAX output rows (pitch ldv) | the OMAX-row count / G^T tile, then
+// one row of context counts per group (same pitch) | per-output-row masks
+// of negative slots | the tile cursor's broadcast word.  At k = 5 and
+// d = 128 the f32 32-row layout (40.5 KB)
+// fits four workgroups in a CU's 160 KB, the bf16 one six.
 struct Lds {
     uint32_t ldv;    // row pitch of staged rows, in elements
     uint32_t wrows;  // rows of per-group context counts
-    size_t o_desc, o_rows, o_g, o_mask, total;
+    size_t o_desc, o_rows, o_g, o_mask, o_grab, total;
 };
 
 __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax, uint32_t k) {
@@ -82,10 +86,11 @@ __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t 
     L.ldv = dp + (esize == 2 ? 16u : 4u);
     L.wrows = tile_wrows(k);
     size_t o = 0;
-    L.o_desc = o; o += kDescBytes;
+    L.o_desc = o; o += 2 * kDescBytes;
     L.o_rows = o; o += (size_t)(rmax + OMAX) * L.ldv * esize;
     L.o_g = o;    o += (size_t)(OMAX + L.wrows) * gt_pitch(esize, rmax) * 4;
     L.o_mask = o; o += (size_t)OMAX * 4;
+    L.o_grab = o; o += 8;
     L.total = o;
     return L;
 }
@@ -106,6 +111,15 @@ __device__ __forceinline__ bf8 join8(s4 a, s4 b) {
 
 __device__ __forceinline__ float load_elem(const float *p) { return *p; }
 __device__ __forceinline__ float load_elem(const uint16_t *p) { return bf16_to_f32(*p); }
+
+// s_waitcnt immediate for vmcnt(0) with expcnt and lgkmcnt left at their
+// maxima (gfx9 encoding: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] |
+// vmcnt[15:14])
+constexpr unsigned kWaitVm0 = 0x0F70u;
+
+// descriptor slots per cursor request of the 32-row launch (the 48/64-row
+// launch takes 64: most of its slots hold small tiles)
+constexpr uint32_t kChunk32 = 16;
 
 // sigma(x) with word2vec's clamp (0 below -MAX_EXP, 1 above), branch-free
 __device__ __forceinline__ float sigmoid_clamped(float x) {
@@ -219,8 +233,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     constexpr int NB = sizeof(T) == 4 ? 8 : 5;        // gathered vectors held per thread
     extern __shared__ __align__(16) unsigned char lds[];
     const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM, a.negative);
-    TileDesc &td = *(TileDesc *)(lds + L.o_desc);
-    const uint32_t *toks = (const uint32_t *)(lds + L.o_desc + offsetof(TileDesc, rin_tok));
     const uint32_t ldv = L.ldv;
     T *Vs = (T *)(lds + L.o_rows);
     T *Us = Vs + RM * ldv;
@@ -239,8 +251,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint32_t round = a.round_state->round;
     const float alpha = a.round_state->alpha;
 
-    // gathers: vector v = tid + kBlock*j of a tile covers row v / vpr and
-    // 16-byte column v % vpr; (row, column) advance by fixed steps
+    // row gathers: 16-byte vectors per row, each thread's first one
     const uint32_t vpr = dp / EPV;
     const uint32_t st_r0 = tid / vpr, st_c0 = tid % vpr;
     const uint32_t st_dr = kBlock / vpr, st_dc = kBlock % vpr;
@@ -251,12 +262,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const float inv_k = k ? 1.f / (float)k : 0.f;
     // f32 write-back tiles start from the staged rows (acc = old + delta)
     constexpr bool acc_old = WB && sizeof(T) == 4 && !DBG;
-    // stochastic-rounding stream of this thread for the launch (bf16 tables)
+    // stochastic-rounding stream of this thread (bf16 tables), seeded per tile
     uint32_t rs = 0;
-    if constexpr (sizeof(T) == 2) {
-        rs = draw(a.seed, a.sr_base + blockIdx.x * kBlock + tid, DOM_ROUND_BF16, 0u, round).x;
-        if (rs == 0) rs = 0x9E3779B9u;
-    }
 
     // bf16 products contract over whole 32-row blocks, reading staged rows
     // past a tile's own (rows of earlier tiles, finite); clear them once
@@ -271,59 +278,84 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint4 *descs = (const uint4 *)a.tiles;
     const uint32_t *desc_words = (const uint32_t *)a.tiles;
 
-    // Tiles of this workgroup: blockIdx.x + i * gridDim.x, skipping slots
-    // without a tile (nc = 0).  Every wave reads the nc words of the next 64
-    // candidates and ballots, so all waves walk the same live tiles with no
-    // barrier spent on empty slots.
-    uint64_t win = blockIdx.x, live = 0;
+    // Tiles come from a cursor shared by the launch: a workgroup takes
+    // chunks of consecutive descriptor slots, so faster workgroups take
+    // more tiles (a workgroup's speed depends on its placement).  Every wave
+    // reads the nc and n_in words of the chunk's slots and ballots, so all
+    // waves walk the same live tiles with no barrier spent on empty slots.
+    // The chunk after the current one is requested ahead by thread 0 and
+    // reaches every wave through LDS at the staging barrier; a chunk with
+    // no tile left before that waits for it at a barrier of its own.  The
+    // last workgroup to finish returns the cursor to zero for the next
+    // launch.
+    const uint32_t chunk = a.nin_lo > 32 ? 64u : kChunk32;
+    unsigned long long *cursor = &a.counters[C_CURSOR];
+    unsigned long long *grab_lds = (unsigned long long *)(lds + L.o_grab);
+    uint64_t win = 0, win_nx = 0, live = 0;
+    unsigned long long grab = 0;  // thread 0: the requested chunk
+    bool pending = false;         // a request not yet seen by every wave
+    // thread 0's request to every wave (a uniform call: two barriers)
+    auto publish = [&]() -> uint64_t {
+        if (tid == 0) *grab_lds = grab;
+        __syncthreads();
+        const uint64_t g = *grab_lds;
+        __syncthreads();
+        return g;
+    };
     auto scan = [&]() {
-        const uint64_t t = win + (uint64_t)lane * gridDim.x;
+        const uint64_t t = win + lane;
         bool ok = false;
-        if (t < n_tiles) {
+        if (lane < chunk && t < n_tiles) {
+            // both words loaded before either is tested (one round trip)
             const uint32_t *dw = desc_words + t * (kDescBytes / 4);
-            const uint32_t nin = dw[6];
-            ok = dw[2] != 0u && nin >= a.nin_lo && nin <= a.nin_hi;  // nc, n_in
+            const uint32_t nc = dw[2], nin = dw[6];
+            ok = (nc != 0u) & (nin >= a.nin_lo) & (nin <= a.nin_hi);
         }
         live = __ballot(ok);
     };
     auto next_live = [&]() -> uint64_t {
         while (live == 0) {
-            win += 64ull * gridDim.x;
+            win = pending ? publish() : win_nx;
+            pending = true;
+            if (tid == 0) grab = atomicAdd(cursor, (unsigned long long)chunk);
             if (win >= n_tiles) return n_tiles;
             scan();
         }
-        const uint64_t t = win + (uint64_t)__builtin_ctzll(live) * gridDim.x;
+        const uint64_t t = win + (uint64_t)__builtin_ctzll(live);
         live &= live - 1;
         return t;
     };
-    if (win < n_tiles) scan();
-    uint64_t tile = win < n_tiles ? next_live() : n_tiles;
-    uint4 pre = make_uint4(0, 0, 0, 0);
-    if (tid < (uint32_t)kDescVec && tile < n_tiles) pre = descs[tile * kDescVec + tid];
+    if (tid == 0) grab = atomicAdd(cursor, (unsigned long long)chunk);
+    win_nx = publish();
 
-    for (uint64_t nxt; tile < n_tiles; tile = nxt) {
-        if (tid < (uint32_t)kDescVec) ((uint4 *)&td)[tid] = pre;
-        for (uint32_t i = tid; i < ((OMAX + L.wrows) * GT + OMAX) / 4; i += kBlock)
-            ((uint4 *)cnt)[i] = make_uint4(0, 0, 0, 0);  // counts, W and the masks
-        __syncthreads();
-        nxt = next_live();
-
-        const uint32_t c0 = td.c0, nc = td.nc, g0 = td.g0, lo = td.lo, npos = td.npos;
-        const uint32_t n_in = td.n_in, n_out = td.n_out;
-        const uint32_t R16 = (n_in + 15) & ~15u, O16 = (n_out + 15) & ~15u;
-        const uint32_t nrows = R16 + O16;
-
-        // ---- 1. gathers of this thread's first NB vectors (padding rows 0):
-        //         slots and LDS reads of the row ids first, then the loads ----
-        // Staged row s < R16 is input row s, s >= R16 output row s - R16 (LDS
-        // row RM + s - R16, table row V + id in the joint allocation).
-        auto advance = [&](uint32_t &r, uint32_t &cc) {
-            r += st_dr;
-            cc += st_dc;
-            if (cc >= vpr) { cc -= vpr; ++r; }
-        };
-        uint4 buf[NB];
-        uint32_t tix[NB], ce[NB];
+    // Software pipeline over the workgroup's tiles: tile i's descriptor is
+    // in slot i & 1, its rows arrive in registers while tile i - 1 trains,
+    // and the descriptor of tile i + 1 arrives with them.  The row gathers
+    // of tile i + 1 thus read the tables before tile i writes them back
+    // (Hogwild, as between concurrent workgroups).
+    auto slot = [&](uint32_t s) -> TileDesc & { return *(TileDesc *)(lds + L.o_desc + s * kDescBytes); };
+    // gathers: vector v = tid + kBlock*j of a tile covers row v / vpr and
+    // 16-byte column v % vpr; (row, column) advance by fixed steps.  Staged
+    // row s < R16 is input row s, s >= R16 output row s - R16 (LDS row
+    // RM + s - R16, table row V + id in the joint allocation).
+    auto advance = [&](uint32_t &r, uint32_t &cc) {
+        r += st_dr;
+        cc += st_dc;
+        if (cc >= vpr) { cc -= vpr; ++r; }
+    };
+    // where this thread's vectors past its NB register ones start (the same
+    // for every tile): larger tiles load those directly when staging
+    uint32_t row_x = st_r0, c_x = st_c0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) advance(row_x, c_x);
+    uint4 buf[NB];
+    // this thread's first NB vectors of the tile in slot s (padding rows 0):
+    // slots and LDS reads of the row ids first, then the loads
+    auto gather = [&](uint32_t s) {
+        const TileDesc &t = slot(s);
+        const uint32_t *tk = t.rin_tok;  // output row ids follow at RMAX
+        const uint32_t n_in = t.n_in, n_out = t.n_out, R16 = (n_in + 15) & ~15u;
+        uint32_t tix[NB], ce[NB], grow[NB];
         uint32_t row = st_r0, c = st_c0;
         // select-only index math (no short-circuit branches): rr wraps for
         // input rows, so rr < n_out holds exactly for live output rows
@@ -335,11 +367,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             ce[j] = c * EPV;
             advance(row, c);
         }
-        uint32_t grow[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const bool live = tix[j] != 0xFFFFFFFFu;
-            const uint32_t id = toks[live ? tix[j] : 0u];
+            const uint32_t id = tk[live ? tix[j] : 0u];
             grow[j] = live ? id + (tix[j] >= (uint32_t)RMAX ? a.V : 0u) : 0u;
         }
 #pragma unroll
@@ -347,10 +378,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             buf[j] = make_uint4(0, 0, 0, 0);
             if (tix[j] != 0xFFFFFFFFu) buf[j] = *(const uint4 *)(syn0 + ((uint64_t)grow[j] * dp + ce[j]));
         }
-        // the next descriptor is consumed one tile later
-        if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
+    };
 
-        // ---- 2. pair counts: P in the low half-word, Q in the high -------------
+    uint64_t tile = next_live();
+    if (tid < (uint32_t)kDescVec && tile < n_tiles) ((uint4 *)&slot(0))[tid] = descs[tile * kDescVec + tid];
+    uint64_t nxt = tile < n_tiles ? next_live() : n_tiles, nxt2;
+    uint4 pre = make_uint4(0, 0, 0, 0);  // the descriptor of tile nxt
+    if (tid < (uint32_t)kDescVec && nxt < n_tiles) pre = descs[nxt * kDescVec + tid];
+    __syncthreads();
+    if (tile < n_tiles) gather(0);
+
+    for (uint32_t s = 0; tile < n_tiles; tile = nxt, nxt = nxt2, s ^= 1) {
+        TileDesc &td = slot(s);
+        const uint32_t *toks = td.rin_tok;
+        nxt2 = nxt < n_tiles ? next_live() : n_tiles;
+        if constexpr (sizeof(T) == 2) {
+            // a tile's rounding bits depend on (round, slot, thread) only,
+            // not on the workgroup that takes it
+            rs = draw(a.seed, tid, DOM_ROUND_BF16, (uint32_t)tile, round).x;
+            if (rs == 0) rs = 0x9E3779B9u;
+        }
+        for (uint32_t i = tid; i < ((OMAX + L.wrows) * GT + OMAX) / 4; i += kBlock)
+            ((uint4 *)cnt)[i] = make_uint4(0, 0, 0, 0);  // counts, W and the masks
+        __syncthreads();
+
+        const uint32_t c0 = td.c0, nc = td.nc, g0 = td.g0, lo = td.lo, npos = td.npos;
+        const uint32_t n_in = td.n_in, n_out = td.n_out;
+        const uint32_t R16 = (n_in + 15) & ~15u, O16 = (n_out + 15) & ~15u;
+        const uint32_t nrows = R16 + O16;
+
+        // ---- 1. pair counts (the rows are in flight): P in the low
+        //         half-word, Q in the high ---------------------------------
         // Q[o][r] = sum over the negative slots q holding row o of W[q/k][r]
         // (pairs of q's group with context row r), less the pairs whose centre
         // row is o itself (word2vec skips a negative equal to the centre).
@@ -380,7 +438,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             if (tid < nslots) atomicOr(&negmask[td.neg_rout[tid]], 1u << tid);
         }
 
-        // ---- 3. gathered vectors to LDS; any further vectors directly ---------
+        // ---- 2. gathered vectors to LDS, any further vectors directly; the
+        //         next tile's descriptor to the other slot ------------------
+        // One wait for every vector-memory operation in flight: the row
+        // loads, the descriptor, and the previous tile's stores.
+        __builtin_amdgcn_s_waitcnt(kWaitVm0);
         {
             uint32_t r = st_r0, cc = st_c0;
 #pragma unroll
@@ -389,16 +451,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 advance(r, cc);
             }
         }
-        while (row < nrows) {
+        for (uint32_t row = row_x, c = c_x; row < nrows; advance(row, c)) {
             const bool in = row < R16;
             const uint32_t r = in ? row : row - R16;
             uint4 val = make_uint4(0, 0, 0, 0);
             if (r < (in ? n_in : n_out))
                 val = *(const uint4 *)((in ? syn0 : syn1) + ((uint64_t)toks[in ? r : RMAX + r] * dp + c * EPV));
             *(uint4 *)(Vs + (in ? row : RM + r) * ldv + c * EPV) = val;
-            advance(row, c);
         }
+        if (tid < (uint32_t)kDescVec) ((uint4 *)&slot(s ^ 1))[tid] = pre;
+        if (pending && tid == 0) *grab_lds = grab;
         __syncthreads();
+        if (pending) {
+            win_nx = *grab_lds;
+            pending = false;
+        }
+
+        // ---- 3. the next tile's rows and the descriptor after it, in flight
+        //         while this tile trains ------------------------------------
+        if (nxt < n_tiles) gather(s ^ 1);
+        pre = make_uint4(0, 0, 0, 0);
+        if (tid < (uint32_t)kDescVec && nxt2 < n_tiles) pre = descs[nxt2 * kDescVec + tid];
 
         const bool dbg_on = DBG && tile < a.dbg_max;
         if (dbg_on) {
@@ -480,7 +553,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         __syncthreads();
 
         if constexpr (sizeof(T) == 2) {
-            // ---- 5/6 on bf16 MFMA (16x16x32): G = hi + lo, two products each ----
+            // ---- 5 on bf16 MFMA (16x16x32): G = hi + lo, two products each ----
             // Operands are swapped (first = table rows, second = G) so the
             // products come out transposed: lane = one row, four columns.
             lds_u8 *l3 = (lds_u8 *)lds;
@@ -553,7 +626,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             continue;
         }
 
-        // ---- 5/6 on f32 MFMA (16x16x4), operands swapped as above -------------
+        // ---- 5 on f32 MFMA (16x16x4), operands swapped as above -------------
         // dVin = G * Uout  (R16 x dp, K = O16: KQ blocks of 16 output rows)
         auto dv_rows = [&](auto kq) {
             constexpr int KQ = decltype(kq)::value;
@@ -638,7 +711,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         ++my_tiles;
         __syncthreads();  // the next tile reuses the LDS
     }
-    if (tid == 0) atomicAdd(&a.counters[C_GROUPS], my_tiles);
+    if (tid == 0) {
+        atomicAdd(&a.counters[C_GROUPS], my_tiles);
+        // every request of this workgroup is complete before it counts as done
+        __threadfence();
+        if (atomicAdd(&a.counters[C_DONE], 1ull) == gridDim.x - 1ull) {
+            atomicExch(cursor, 0ull);
+            atomicExch(&a.counters[C_DONE], 0ull);
+        }
+    }
     for (int o = 32; o > 0; o >>= 1) my_pairs += __shfl_xor(my_pairs, o);
     if (lane == 0) atomicAdd(&a.counters[C_PAIRS], my_pairs);
 }
@@ -710,12 +791,10 @@ hipError_t launch_split(felog_ctx *c, Params a) {
     unsigned blocks = 0;
     a.nin_lo = 0;
     a.nin_hi = 32;
-    a.sr_base = 0;
     hipError_t e = launch_rm<T, 32>(c, a, c->tile_gpc, &blocks);
     if (e != hipSuccess) return e;
     a.nin_lo = 33;
     a.nin_hi = RMAX;
-    a.sr_base = blocks * kBlock;
     return launch_big<T>(c, a, 0, &blocks);
 }
 
@@ -748,7 +827,6 @@ hipError_t launch_tile(felog_ctx *c, const Params &a0) {
     Params a = a0;
     a.nin_lo = 0;
     a.nin_hi = RMAX;
-    a.sr_base = 0;
     unsigned blocks = 0;
     return bf ? launch_big<uint16_t>(c, a, c->tile_gpc, &blocks) : launch_big<float>(c, a, c->tile_gpc, &blocks);
 }
