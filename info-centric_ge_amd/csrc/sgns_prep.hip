@@ -40,9 +40,52 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     const uint64_t n_sent = a.round_state->n_active;
     const uint32_t round = a.round_state->round;
 
-    for (uint64_t sid = (uint64_t)blockIdx.x * kPrepWaves + wv; sid < n_sent; sid += nwaves) {
-        const uint64_t b0 = a.off[sid];
-        const uint32_t n = (uint32_t)(a.off[sid + 1] - b0);
+    // Each wave walks its sentences one ahead: a sentence's offsets and
+    // first 64 tokens are loaded while the one before it is processed, and
+    // its keep thresholds are loaded together with the negatives of its
+    // first groups (so one memory round trip per sentence is exposed).
+    uint64_t sid = (uint64_t)blockIdx.x * kPrepWaves + wv;
+    uint64_t b0 = 0, b0_nx = 0;
+    uint32_t n = 0, n_nx = 0, t_first = 0;
+    if (sid < n_sent) {
+        b0 = a.off[sid];
+        n = (uint32_t)(a.off[sid + 1] - b0);
+        if (lane < n) t_first = a.tok[b0 + lane];
+    }
+    if (sid + nwaves < n_sent) {
+        b0_nx = a.off[sid + nwaves];
+        n_nx = (uint32_t)(a.off[sid + nwaves + 1] - b0_nx);
+    }
+    uint64_t b0_nn = 0;
+    uint32_t n_nn = 0, t_nx = 0;
+    auto advance = [&]() {
+        b0 = b0_nx;
+        n = n_nx;
+        t_first = t_nx;
+        b0_nx = b0_nn;
+        n_nx = n_nn;
+    };
+    for (; sid < n_sent; sid += nwaves, advance()) {
+        // negatives of the first groups (lanes < OMAX: negative j = lane % k
+        // of group lane / k), the same draws as the tile loop's for g0 = 0
+        uint32_t neg_ry = 0, neg_bkt = 0;
+        uint2 neg_e = make_uint2(0, 0);
+        if (k && lane < (uint32_t)OMAX) {
+            const u32x4 r = draw(a.seed, lane / k, DOM_NEGATIVE | (lane % k), (uint32_t)sid, round);
+            neg_bkt = alias_bucket(r.x, a.V);
+            neg_ry = r.y;
+            neg_e = a.uni[neg_bkt];
+        }
+        const uint32_t thr_first = lane < n ? a.keep_thr[t_first] : 0u;
+        // the next sentence's first tokens and the offsets of the one after
+        t_nx = 0;
+        if (sid + nwaves < n_sent && lane < n_nx) t_nx = a.tok[b0_nx + lane];
+        b0_nn = 0;
+        n_nn = 0;
+        if (sid + 2 * nwaves < n_sent) {
+            b0_nn = a.off[sid + 2 * nwaves];
+            n_nn = (uint32_t)(a.off[sid + 2 * nwaves + 1] - b0_nn);
+        }
         // ---- filter into stok[0, m) ----
         uint32_t m = 0, words = 0;
         for (uint32_t chunk = 0; chunk < n; chunk += 64) {
@@ -50,14 +93,14 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             int status = 0;  // 0 none, 1 drop, 2 keep, 3 break
             uint32_t t = 0;
             if (p < n) {
-                t = a.tok[b0 + p];
+                // the draw first: it does not depend on the loads
+                const uint32_t u = draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, round).x;
+                t = chunk == 0 ? t_first : a.tok[b0 + p];
                 // threshold 0: outside the vocabulary (or never kept)
-                const uint32_t thr = a.keep_thr[t];
+                const uint32_t thr = chunk == 0 ? thr_first : a.keep_thr[t];
                 if (a.compat_break && (a.in_vocab[t] == 0 || a.vocab_rank[t] == 0)) status = 3;
                 else if (thr == 0u) status = 1;
-                else
-                    status = (thr == 0xFFFFFFFFu ||
-                              draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, round).x < thr) ? 2 : 1;
+                else status = (thr == 0xFFFFFFFFu || u < thr) ? 2 : 1;
             }
             const uint64_t brk_bal = __ballot(status == 3);
             const uint32_t brk = brk_bal ? chunk + (uint32_t)__ffsll((long long)brk_bal) - 1 : 0xFFFFFFFFu;
@@ -104,13 +147,17 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 if (lane == 0) slot = n_sent + atomicAdd(&a.counters[C_TILES], 1ull);
                 slot = __shfl(slot, 0);
             }
-            // negatives of the tile's groups
+            // negatives of the tile's groups (the first tile's drawn above)
             if (lane < nneg) {
-                const uint32_t gq = g0 + lane / k, j = lane % k;
-                const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, round);
-                const uint32_t bkt = alias_bucket(r.x, a.V);
-                const uint2 e = a.uni[bkt];
-                sneg[lane] = r.y < e.x ? bkt : e.y;
+                if (g0 == 0) {
+                    sneg[lane] = neg_ry < neg_e.x ? neg_bkt : neg_e.y;
+                } else {
+                    const uint32_t gq = g0 + lane / k, j = lane % k;
+                    const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, round);
+                    const uint32_t bkt = alias_bucket(r.x, a.V);
+                    const uint2 e = a.uni[bkt];
+                    sneg[lane] = r.y < e.x ? bkt : e.y;
+                }
             }
             __builtin_amdgcn_wave_barrier();
             if (slot >= a.tiles_cap) {

@@ -214,7 +214,8 @@ __global__ void k_gather_lens(const uint32_t *__restrict__ active, const RoundSt
         off[1 + i] = i < n ? lens[active[i]] : 0;
 }
 
-// One wavefront per walk: lanes copy consecutive vertices of the path.
+// Sixteen lanes per walk (four walks per wavefront, so four walks' loads
+// are in flight at once): lanes copy consecutive vertices of the path.
 // Thread 0 also records the round's token count.
 __global__ void k_compact(const uint32_t *__restrict__ active, RoundState *__restrict__ rs,
                           uint64_t *__restrict__ off, const uint32_t *__restrict__ slots,
@@ -223,16 +224,15 @@ __global__ void k_compact(const uint32_t *__restrict__ active, RoundState *__res
                           uint32_t *__restrict__ sent_id) {
     const uint32_t n = rs->n_active;
     if (blockIdx.x == 0 && threadIdx.x == 0) rs->n_tokens = off[n];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t i = wave; i < n; i += nwaves) {
+    const uint32_t l = threadIdx.x & 15;
+    const uint32_t group = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const uint32_t ngroups = (gridDim.x * blockDim.x) >> 4;
+    for (uint32_t i = group; i < n; i += ngroups) {
         const uint32_t w = active[i];
         const uint64_t b = off[i], len = off[i + 1] - b;
-        if (lane == 0) sent_id[i] = w;
-        const uint64_t ob = ovf_base[w];
-        for (uint64_t k = lane; k < len; k += 64)
-            tok[b + k] = k < slot ? slots[(uint64_t)w * slot + k] : ovf[ob + (k - slot)];
+        if (l == 0) sent_id[i] = w;
+        for (uint64_t k = l; k < len; k += 16)
+            tok[b + k] = k < slot ? slots[(uint64_t)w * slot + k] : ovf[ovf_base[w] + (k - slot)];
     }
 }
 
@@ -380,7 +380,7 @@ hipError_t compact_enqueue(felog_ctx *c) {
     HIP_TRY(hipGetLastError());
     size_t tmp = c->tmp_bytes;
     HIP_TRY(hipcub::DeviceScan::InclusiveSum(c->d_tmp, tmp, c->d_off + 1, c->d_off + 1, (uint64_t)W, c->stream));
-    k_compact<<<grid_for((uint64_t)W * 64, 256, 1u << 16), 256, 0, c->stream>>>(
+    k_compact<<<grid_for((uint64_t)W * 16, 256, 1u << 16), 256, 0, c->stream>>>(
         c->d_active, c->d_rs, c->d_off, c->d_slots, c->slot, c->d_ovf_base, c->d_ovf, c->d_tok, c->d_sent_id);
     return hipGetLastError();
 }
