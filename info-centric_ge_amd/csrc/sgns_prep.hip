@@ -27,6 +27,23 @@ namespace {
 
 constexpr int kPrepWaves = kBlock / 64;
 
+// Lowest lane whose valid value equals this lane's (the lane itself for
+// an invalid lane): one uniform pass per distinct value, each a broadcast
+// of the lowest unmatched lane's value and a ballot of the lanes equal to it.
+__device__ __forceinline__ uint32_t first_lane_of(uint32_t t, bool valid) {
+    uint32_t first = threadIdx.x & 63;
+    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+    uint64_t todo = vmask;
+    while (todo) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t tq = __builtin_amdgcn_readlane(t, q);
+        const uint64_t eq = __builtin_amdgcn_ballot_w64(t == tq) & vmask;
+        if (valid && t == tq) first = q;
+        todo &= ~eq;
+    }
+    return first;
+}
+
 __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     __shared__ uint32_t s_tok[kPrepWaves][MAXKEEP + 8];
     __shared__ uint32_t s_neg[kPrepWaves][OMAX];
@@ -169,13 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             {
                 const bool valid = lane < npos;
                 const uint32_t t = valid ? stok[lo + lane] : 0xFFFFFFFFu;
-                // first lane holding the same row: a uniform loop of lane
-                // broadcasts (no memory round trip per comparison)
-                uint32_t first = lane;
-                for (uint32_t q = 0; q < npos; ++q) {
-                    const uint32_t tq = __builtin_amdgcn_readlane(t, q);
-                    if (q < first && tq == t) first = q;
-                }
+                const uint32_t first = first_lane_of(t, valid);
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
@@ -196,11 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 const bool valid = lane < nl;
                 uint32_t t = 0xFFFFFFFFu;
                 if (valid) t = lane < nc ? stok[c0 + lane] : sneg[lane - nc];
-                uint32_t first = lane;
-                for (uint32_t q = 0; q < nl; ++q) {
-                    const uint32_t tq = __builtin_amdgcn_readlane(t, q);
-                    if (q < first && tq == t) first = q;
-                }
+                const uint32_t first = first_lane_of(t, valid);
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
