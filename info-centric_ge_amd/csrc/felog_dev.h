@@ -31,13 +31,15 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
 }
 
 // Counter layout {x: index, y: domain | sub-index, z: stream, w: round}.
-// Domains keep the walk, subsampling, window, negative, rounding and
-// evaluation streams disjoint under one key.
+// Domains keep the walk, SGNS (subsampling, window, negatives), rounding,
+// evaluation and budget-network streams disjoint under one key.
 enum : uint32_t {
     DOM_WALK = 0x00000000u,       // y = resample attempt, x = step, z = walker
-    DOM_SUBSAMPLE = 0x10000000u,  // x = raw position, z = sentence
-    DOM_WINDOW = 0x20000000u,     // x = kept position of the centre, z = sentence
-    DOM_NEGATIVE = 0x30000000u,   // y |= negative slot, x = group, z = sentence
+    // x = index, z = sentence; of the four output words, the first is the
+    // subsample coin of raw position x, the second the window shrink of kept
+    // centre x, the last two negative j of group g for x = g * negative + j
+    // (bucket, coin)
+    DOM_SGNS = 0x10000000u,
     DOM_EVAL = 0x40000000u,       // y |= draw index, x = vertex, z = ~0
     DOM_ROUND_BF16 = 0x50000000u, // x = thread, z = tile slot (seeds xorshift32)
     DOM_MLP = 0x60000000u,        // x = sample slot of a budget-MLP step, w = step

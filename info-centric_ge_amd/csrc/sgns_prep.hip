@@ -83,16 +83,16 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
         n_nx = n_nn;
     };
     for (; sid < n_sent; sid += nwaves, advance()) {
-        // negatives of the first groups (lanes < OMAX: negative j = lane % k
-        // of group lane / k), the same draws as the tile loop's for g0 = 0
-        uint32_t neg_ry = 0, neg_bkt = 0;
+        // The sentence's draw for index lane: the subsample coin of raw
+        // position lane, the window shrink of kept centre lane and negative
+        // lane % k of group lane / k (the first tile's, whose index is the
+        // lane).  The alias entries of the negatives the first tile can hold
+        // (k per group of at most ceil(n / B) groups) load with the
+        // thresholds: each is a random line of the table.
+        const u32x4 r0 = draw(a.seed, lane, DOM_SGNS, (uint32_t)sid, round);
+        const uint32_t neg_bkt = alias_bucket(r0.z, a.V);
         uint2 neg_e = make_uint2(0, 0);
-        if (k && lane < (uint32_t)OMAX) {
-            const u32x4 r = draw(a.seed, lane / k, DOM_NEGATIVE | (lane % k), (uint32_t)sid, round);
-            neg_bkt = alias_bucket(r.x, a.V);
-            neg_ry = r.y;
-            neg_e = a.uni[neg_bkt];
-        }
+        if (lane < (uint32_t)OMAX && lane < k * ((n + B - 1) / B)) neg_e = a.uni[neg_bkt];
         const uint32_t thr_first = lane < n ? a.keep_thr[t_first] : 0u;
         // the next sentence's first tokens and the offsets of the one after
         t_nx = 0;
@@ -110,8 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             int status = 0;  // 0 none, 1 drop, 2 keep, 3 break
             uint32_t t = 0;
             if (p < n) {
-                // the draw first: it does not depend on the loads
-                const uint32_t u = draw(a.seed, p, DOM_SUBSAMPLE, (uint32_t)sid, round).x;
+                const uint32_t u = chunk == 0 ? r0.x : draw(a.seed, p, DOM_SGNS, (uint32_t)sid, round).x;
                 t = chunk == 0 ? t_first : a.tok[b0 + p];
                 // threshold 0: outside the vocabulary (or never kept)
                 const uint32_t thr = chunk == 0 ? thr_first : a.keep_thr[t];
@@ -167,13 +166,13 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             // negatives of the tile's groups (the first tile's drawn above)
             if (lane < nneg) {
                 if (g0 == 0) {
-                    sneg[lane] = neg_ry < neg_e.x ? neg_bkt : neg_e.y;
+                    sneg[lane] = r0.w < neg_e.x ? neg_bkt : neg_e.y;
                 } else {
-                    const uint32_t gq = g0 + lane / k, j = lane % k;
-                    const u32x4 r = draw(a.seed, gq, DOM_NEGATIVE | j, (uint32_t)sid, round);
-                    const uint32_t bkt = alias_bucket(r.x, a.V);
+                    // negative lane % k of group g0 + lane / k: index g0 * k + lane
+                    const u32x4 r = draw(a.seed, g0 * k + lane, DOM_SGNS, (uint32_t)sid, round);
+                    const uint32_t bkt = alias_bucket(r.z, a.V);
                     const uint2 e = a.uni[bkt];
-                    sneg[lane] = r.y < e.x ? bkt : e.y;
+                    sneg[lane] = r.w < e.x ? bkt : e.y;
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -220,8 +219,10 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                     if (lane >= n_out) td->rout_tok[lane] = 0xFFFFFFFFu;
                     td->cen_rout[lane] = (valid && lane < nc) ? (uint8_t)my : 0;
                     td->neg_rout[lane] = lane < nneg ? (uint8_t)my_neg : 0;
-                    td->bshrink[lane] = lane < nc
-                        ? (uint8_t)(draw(a.seed, c0 + lane, DOM_WINDOW, (uint32_t)sid, round).x % w) : 0;
+                    // window shrink of kept centre c0 + lane (the first tile's: r0)
+                    uint32_t by = r0.y;
+                    if (c0 != 0) by = draw(a.seed, c0 + lane, DOM_SGNS, (uint32_t)sid, round).y;
+                    td->bshrink[lane] = lane < nc ? (uint8_t)(by % w) : 0;
                 }
                 if (lane == 0) td->n_out = n_out;
             }

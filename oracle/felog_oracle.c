@@ -34,9 +34,10 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* counter domains (y word) */
 #define D_WALK 0x00000000u
-#define D_SUBSAMPLE 0x10000000u
-#define D_WINDOW 0x20000000u
-#define D_NEGATIVE 0x30000000u
+/* One SGNS draw per (index x, sentence z): word 0 subsamples raw position
+ * x, word 1 is the window shrink of kept centre x, words 2-3 draw negative
+ * j of group g for x = g * negative + j (bucket, coin). */
+#define D_SGNS 0x10000000u
 #define D_EVAL 0x40000000u
 
 static void rnd(uint64_t seed, uint32_t x, uint32_t y, uint32_t z, uint32_t w, uint32_t out[4]) {
@@ -438,7 +439,7 @@ static double sigmoid_clamped(double x) {
 }
 
 /* Sentence filter shared by the SGNS oracle: vocabulary, subsampling from
- * Philox(seed; position, SUBSAMPLE, sentence, round), 1000-token cut. */
+ * word 0 of Philox(seed; position, SGNS, sentence, round), 1000-token cut. */
 static uint32_t filter_sentence(const orc_sgns_params *p, const uint32_t *t, uint32_t n, uint64_t sid, uint32_t *K) {
     uint32_t m = 0;
     for (uint32_t i = 0; i < n && m < MAXKEEP; ++i) {
@@ -449,7 +450,7 @@ static uint32_t filter_sentence(const orc_sgns_params *p, const uint32_t *t, uin
         const uint32_t thr = p->keep_thr[x];
         if (thr != 0xFFFFFFFFu) {
             uint32_t r[4];
-            rnd(p->seed, i, D_SUBSAMPLE, (uint32_t)sid, p->round, r);
+            rnd(p->seed, i, D_SGNS, (uint32_t)sid, p->round, r);
             if (!(r[0] < thr)) continue;
         }
         K[m++] = x;
@@ -493,14 +494,15 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
             uint32_t bshr[OMAX], neg[OMAX], rin_tok[RMAX], rout_tok[OMAX], pos_rin[RMAX], cen_rout[OMAX], neg_rout[OMAX];
             for (uint32_t i = 0; i < nc; ++i) {
                 uint32_t r[4];
-                rnd(p->seed, c0 + i, D_WINDOW, (uint32_t)sid, p->round, r);
-                bshr[i] = r[0] % w;
+                rnd(p->seed, c0 + i, D_SGNS, (uint32_t)sid, p->round, r);
+                bshr[i] = r[1] % w;
             }
             for (uint32_t q = 0; q < nneg; ++q) {
                 uint32_t r[4];
-                rnd(p->seed, g0 + q / k, D_NEGATIVE | (q % k), (uint32_t)sid, p->round, r);
-                const uint32_t bk = mulhi32(r[0], p->V);
-                neg[q] = r[1] < p->uni_thr[bk] ? bk : p->uni_alias[bk];
+                /* negative q % k of group g0 + q / k: index g0 * k + q */
+                rnd(p->seed, g0 * k + q, D_SGNS, (uint32_t)sid, p->round, r);
+                const uint32_t bk = mulhi32(r[2], p->V);
+                neg[q] = r[3] < p->uni_thr[bk] ? bk : p->uni_alias[bk];
             }
             uint32_t n_in = 0, n_out = 0;
             for (uint32_t q = lo; q < hi; ++q) {

@@ -366,11 +366,11 @@ def test_sgns_oracle_gradient_finite_difference():
     neg = {}
     for g in range(ng):
         for j in range(k):
-            r = orc.philox([g, 0x30000000 | j, 0, 1], [77, 0])
-            bk = (int(r[0]) * V) >> 32
-            neg[(g, j)] = bk if int(r[1]) < int(ut[bk]) else int(ua[bk])
+            r = orc.philox([g * k + j, 0x10000000, 0, 1], [77, 0])
+            bk = (int(r[2]) * V) >> 32
+            neg[(g, j)] = bk if int(r[3]) < int(ut[bk]) else int(ua[bk])
     for c in range(m):
-        b = int(orc.philox([c, 0x20000000, 0, 1], [77, 0])[0]) % w
+        b = int(orc.philox([c, 0x10000000, 0, 1], [77, 0])[1]) % w
         for aa in range(b, 2 * w + 1 - b):
             p = c - w + aa
             if aa == w or p < 0 or p >= m:
