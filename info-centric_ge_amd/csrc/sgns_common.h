@@ -35,8 +35,9 @@ struct TileDesc {
     uint8_t cen_rout[OMAX];   // centre c0+i -> output row
     uint8_t neg_rout[OMAX];   // negative q of the tile -> output row
     uint8_t bshrink[OMAX];    // window shrink b of centre c0+i
+    uint8_t cen_coll[OMAX];   // negatives of centre c0+i's group equal to it
 };
-static_assert(sizeof(TileDesc) == 608, "descriptor layout");
+static_assert(sizeof(TileDesc) == 640, "descriptor layout");
 constexpr int kDescBytes = 640;
 constexpr int kDescVec = kDescBytes / 16;
 

@@ -214,6 +214,17 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 const uint32_t n_out = (uint32_t)__popcll(bal);
                 // output row of negative q sits on lane nc + q
                 const uint32_t my_neg = __shfl(my, (int)((nc + lane) & 63));
+                // negatives of a centre's group equal to the centre (word2vec
+                // skips them): its group is lane / B within the tile (c0 is a
+                // group boundary; the float quotient of integers below 64 is
+                // exact after truncation), its negatives sit on lanes
+                // nc + group * k + j
+                uint32_t coll = 0;
+                {
+                    const uint32_t grp = (uint32_t)((float)lane / (float)B);
+                    const uint32_t src = nc + grp * k;
+                    for (uint32_t j = 0; j < k; ++j) coll += __shfl(t, (int)((src + j) & 63)) == t;
+                }
                 if (lane < OMAX) {
                     if (isf) td->rout_tok[idx] = t;
                     if (lane >= n_out) td->rout_tok[lane] = 0xFFFFFFFFu;
@@ -223,6 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                     uint32_t by = r0.y;
                     if (c0 != 0) by = draw(a.seed, c0 + lane, DOM_SGNS, (uint32_t)sid, round).y;
                     td->bshrink[lane] = lane < nc ? (uint8_t)(by % w) : 0;
+                    td->cen_coll[lane] = lane < nc ? (uint8_t)coll : 0;
                 }
                 if (lane == 0) td->n_out = n_out;
             }

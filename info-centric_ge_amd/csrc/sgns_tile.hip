@@ -419,8 +419,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             const uint32_t b = td.bshrink[ci];
             const uint32_t ro = td.cen_rout[ci];
             const uint32_t grp = (c0 + ci) / B - g0;  // this centre's group within the tile
-            uint32_t coll = 0;
-            for (uint32_t j = 0; j < k; ++j) coll += td.neg_rout[grp * k + j] == ro;
+            const uint32_t coll = td.cen_coll[ci];     // its group's negatives equal to it
             const uint32_t pinc = 1u - (coll << 16);
             uint32_t *wrow = wcnt + grp * GT;
             for (uint32_t aa = cs; aa < span; aa += 8) {
