@@ -341,6 +341,38 @@ def test_init_syn0_lcg_closed_form():
             assert got[order[a], b] == expect
 
 
+# ---- SGNS subsampling: the kept positions against Philox word 0
+def test_sgns_oracle_subsample_draws():
+    """Sentences of distinct tokens short enough for one tile: the tile's
+    input rows are the kept tokens in order, which must be exactly the
+    positions i whose draw (i, SGNS domain, sentence, round) has first word
+    below the keep threshold (here 2^31: keep with probability 1/2)."""
+    V, seed, rnd = 64, 0x1234_0000_0077, 3
+    deg = np.full(V, 10, np.uint32)
+    order, rank, inv, vs, tw = orc.vocab(deg, 5)
+    ut, ua = orc.unigram(deg, inv)
+    thr = 1 << 31
+    tables = dict(in_vocab=inv, vocab_rank=rank, keep_thr=np.full(V, thr, np.uint32), uni_thr=ut, uni_alias=ua)
+    rng = np.random.default_rng(5)
+    sents = [rng.permutation(V)[:20].astype(np.uint32) for _ in range(6)]
+    offsets = np.concatenate([[0], np.cumsum([len(x) for x in sents])]).astype(np.uint64)
+    tokens = np.concatenate(sents)
+    d = 4
+    ref = orc.sgns_groups(tables, offsets, tokens, np.zeros((V, d)), np.zeros((V, d)), d=d, window=3, negative=2,
+                          batch=0, alpha=0.0, rnd=rnd, seed=seed)
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    got = {int(g[0]): r[:int(g[3])].tolist() for g, r in zip(ref["groups"], ref["rows"])}
+    n_checked = 0
+    for sid, sent in enumerate(sents):
+        kept = [int(t) for i, t in enumerate(sent) if int(orc.philox([i, 0x10000000, sid, rnd], key)[0]) < thr]
+        if len(kept) < 2:
+            assert sid not in got
+            continue
+        assert got[sid] == kept, sid
+        n_checked += 1
+    assert n_checked >= 4
+
+
 # ---- SGNS FP64 oracle: gradients by finite differences of the SGNS loss
 def test_sgns_oracle_gradient_finite_difference():
     rng = np.random.default_rng(8)
