@@ -295,8 +295,16 @@ felog_status felog_budget_copy_out(felog_ctx *ctx, uint16_t *count,
                                    uint16_t *length, uint32_t *rounds);
 /* Rounds of the budgeted run: walks and training until every budgeted
  * vertex has used its count and (when plain-rule vertices remain) the plain
- * stop rule fires; evaluation runs only for plain-rule vertices. */
+ * stop rule fires; evaluation runs only for plain-rule vertices (and, with
+ * the recheck on, once for the budgeted vertices at min_R). */
 felog_status felog_run_budget(felog_ctx *ctx);
+/* No counterpart in the reference.  on != 0: at round min_R the budgeted
+ * vertices whose count ends there are scored once with the plain rule's
+ * convergence test (src/word2vec.cu:2016-2080, mean cosine to at most 30
+ * neighbours > 0.65); each that fails walks one more round (its count
+ * becomes min_R + 1).  Their flags and the plain rule's stop count are left
+ * alone.  Off by default; applies to felog_run_budget and felog_run_graph. */
+felog_status felog_budget_recheck(felog_ctx *ctx, int on);
 
 /* ---- visit diagnostics (include/walk.hpp:754-788) ---------------------- */
 /* Start (or restart) counting corpus visits; each later walk round adds its

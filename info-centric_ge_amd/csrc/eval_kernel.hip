@@ -7,7 +7,9 @@
 // otherwise 30 distinct neighbours drawn with Floyd's algorithm from Philox
 // (the reference shuffles with a std::random_device seed, :1787-1792).
 // Scores above 0.65 (:46) set the vertex's flag, after which no walker
-// starts there.
+// starts there.  The budget recheck runs the same test once, at min_R, on
+// the budgeted vertices whose count ends there: each that fails gets one
+// more round; flags and the stop rule's counts are left alone.
 #include <hip/hip_runtime.h>
 #include "felog_dev.h"
 #include "felog_internal.h"
@@ -38,18 +40,25 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                                               uint8_t *__restrict__ flags, float *__restrict__ scores,
                                               const T *__restrict__ syn0, uint32_t V, uint32_t dp,
                                               uint32_t d, const RoundState *__restrict__ rs, int gated,
-                                              uint64_t seed, const uint16_t *__restrict__ budget_cnt,
+                                              int recheck, uint64_t seed, uint16_t *__restrict__ budget_cnt,
                                               unsigned long long *counters) {
     const RoundState s = *rs;
-    if (gated && !round_evaluates(s)) return;
+    if (gated) {
+        // the recheck belongs to round min_R, when that round walked
+        const bool due = recheck ? s.round == s.eval_from && !(s.budget && s.n_active == 0) : round_evaluates(s);
+        if (!due) return;
+    }
     const uint32_t round = s.round;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long n_eval = 0, n_conv = 0;
     for (uint32_t v = wave; v < V; v += nwaves) {
-        // vertices with a learned walk budget are not judged by convergence
-        if (!in_vocab[v] || flags[v] || (budget_cnt && budget_cnt[v])) continue;
+        // vertices with a learned walk budget are not judged by convergence,
+        // except by the recheck when their count ends this round
+        if (!in_vocab[v] || flags[v]) continue;
+        const uint32_t bc = budget_cnt ? budget_cnt[v] : 0u;
+        if (recheck ? bc != round : bc != 0u) continue;
         const uint64_t b = row_ptr[v];
         const uint32_t deg = (uint32_t)(row_ptr[v + 1] - b);
         float score = 0.f;
@@ -92,12 +101,16 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
             score = sum / (float)cnt;
         }
         if (lane == 0) {
-            scores[v] = score;
-            if (score > kConverge) {
-                flags[v] = 1;
-                ++n_conv;
+            if (recheck) {
+                if (!(score > kConverge)) budget_cnt[v] = (uint16_t)(round + 1);
             } else {
-                ++n_eval;
+                scores[v] = score;
+                if (score > kConverge) {
+                    flags[v] = 1;
+                    ++n_conv;
+                } else {
+                    ++n_eval;
+                }
             }
         }
     }
@@ -115,20 +128,27 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
         if (e_ != hipSuccess) return e_;        \
     } while (0)
 
-hipError_t eval_enqueue(felog_ctx *c, bool gated) {
+hipError_t eval_launch(felog_ctx *c, bool gated, bool recheck) {
     const unsigned blocks = c->cu_count * 8;
-    const uint16_t *bcnt = c->budget_on ? c->d_budget_cnt : nullptr;
+    uint16_t *bcnt = c->budget_on ? c->d_budget_cnt : nullptr;
     if (c->opt.precision == FELOG_BF16)
         k_eval<uint16_t><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
                                                        c->d_scores, (const uint16_t *)c->d_syn0, c->V, c->dp,
-                                                       c->d, c->d_rs, gated ? 1 : 0, c->opt.seed, bcnt,
-                                                       (unsigned long long *)c->d_counters);
+                                                       c->d, c->d_rs, gated ? 1 : 0, recheck ? 1 : 0,
+                                                       c->opt.seed, bcnt, (unsigned long long *)c->d_counters);
     else
         k_eval<float><<<blocks, 256, 0, c->stream>>>(c->d_row_ptr, c->d_col, c->d_in_vocab, c->d_flags,
                                                     c->d_scores, (const float *)c->d_syn0, c->V, c->dp, c->d,
-                                                    c->d_rs, gated ? 1 : 0, c->opt.seed, bcnt,
+                                                    c->d_rs, gated ? 1 : 0, recheck ? 1 : 0, c->opt.seed, bcnt,
                                                     (unsigned long long *)c->d_counters);
     return hipGetLastError();
+}
+
+hipError_t eval_enqueue(felog_ctx *c, bool gated) { return eval_launch(c, gated, false); }
+
+hipError_t recheck_enqueue(felog_ctx *c, bool gated) {
+    if (!c->budget_on) return hipSuccess;
+    return eval_launch(c, gated, true);
 }
 
 // Eager evaluation of the round the round state holds.

@@ -492,6 +492,13 @@ felog_status felog_budget_features(felog_ctx *c, float *features) {
 }
 
 namespace {
+// The last round a budgeted vertex can walk: the largest count, or min_R + 1
+// when the recheck may extend counts that end at min_R.
+uint32_t last_budget_round(const felog_ctx *c) {
+    const uint32_t r = c->budget_rounds;
+    return c->budget_recheck && c->opt.min_R + 1 > r ? c->opt.min_R + 1 : r;
+}
+
 // Device copies of a training set and the optimiser state; freed on return.
 struct MlpBufs {
     float *x = nullptr, *t = nullptr, *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
@@ -611,6 +618,15 @@ felog_status felog_budget_set(felog_ctx *c, const felog_budget_model *model) {
     return FELOG_OK;
 }
 
+felog_status felog_budget_recheck(felog_ctx *c, int on) {
+    if (!c) return FELOG_ERR_ARG;
+    const bool v = on != 0;
+    // the captured round holds the recheck launch or not
+    if (v != c->budget_recheck) drop_round_graph(c);
+    c->budget_recheck = v;
+    return FELOG_OK;
+}
+
 felog_status felog_budget_copy_out(felog_ctx *c, uint16_t *count, uint16_t *length, uint32_t *rounds) {
     felog_status s = need_built(c);
     if (s) return s;
@@ -627,13 +643,18 @@ felog_status felog_run_budget(felog_ctx *c) {
     if (!c->budget_on) return fail(c, FELOG_ERR_STATE, "no budget installed");
     // plain-rule vertices: those with count 0
     bool plain_stopped = !c->budget_plain_left;
+    const uint32_t last = last_budget_round(c);
     for (uint32_t r = c->rounds_done + 1;; ++r) {
         if (c->opt.max_rounds && r > c->opt.max_rounds) break;
-        if (r > c->budget_rounds && plain_stopped) break;
+        if (r > last && plain_stopped) break;
         felog_corpus corpus;
         if ((s = felog_walk_round(c, r, &corpus))) return s;
         if (corpus.n_walks == 0) break;
         if ((s = felog_train_round(c, &corpus))) return s;
+        if (c->budget_recheck && r == c->opt.min_R) {
+            TRY(c, recheck_enqueue(c, false));
+            TRY(c, hipStreamSynchronize(c->stream));
+        }
         if (!plain_stopped && r >= c->opt.min_R) {
             uint32_t nu = 0;
             int stop = 0;
@@ -664,6 +685,7 @@ hipError_t capture_round(felog_ctx *c, bool visits) {
     if (e == hipSuccess && visits) e = add_round_visits(c);
     if (e == hipSuccess) e = sgns_enqueue(c, c->d_off, c->d_tok, nullptr, nullptr);
     if (e == hipSuccess) e = eval_enqueue(c, true);
+    if (e == hipSuccess && c->budget_on && c->budget_recheck) e = recheck_enqueue(c, true);
     if (e == hipSuccess) e = round_end(c);
     const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e == hipSuccess) e = e2;
@@ -690,6 +712,7 @@ felog_status felog_run_graph(felog_ctx *c, uint32_t max_rounds) {
         c->round_exec_visits = visits;
     }
     bool plain_stopped = budget && !c->budget_plain_left;
+    const uint32_t last = budget ? last_budget_round(c) : 0;
     RoundState &h = *c->h_rs;
     std::memset(&h, 0, sizeof(h));
     h.round = c->rounds_done;
@@ -702,7 +725,7 @@ felog_status felog_run_graph(felog_ctx *c, uint32_t max_rounds) {
     for (uint32_t r = first;; ++r) {
         if (c->opt.max_rounds && r > c->opt.max_rounds) break;
         if (max_rounds && r - first >= max_rounds) break;
-        if (budget && r > c->budget_rounds && plain_stopped) break;
+        if (budget && r > last && plain_stopped) break;
         TRY(c, hipEventRecord(c->ev0, c->stream));
         TRY(c, hipGraphLaunch(c->round_exec, c->stream));
         TRY(c, hipEventRecord(c->ev1, c->stream));

@@ -158,6 +158,7 @@ struct felog_ctx {
     float *d_feat = nullptr;             // V x FELOG_BUDGET_FEATURES
     uint32_t budget_rounds = 0;          // largest predicted count
     bool budget_plain_left = false;      // some vertex keeps the plain rule (count 0)
+    bool budget_recheck = false;         // score budgeted vertices once at min_R
     // corpus visit histogram (cumulative over walk rounds while tracked)
     uint32_t *d_visits = nullptr;        // V
     uint64_t visits_total = 0;
@@ -230,6 +231,9 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
 // eval_kernel.hip
 // gated: evaluate only when the round state says so (captured rounds)
 hipError_t eval_enqueue(felog_ctx *c, bool gated);
+// the budget recheck at min_R (gated: only when the round state's round is
+// min_R and the round walked)
+hipError_t recheck_enqueue(felog_ctx *c, bool gated);
 hipError_t run_eval(felog_ctx *c, float *ms);
 // budget_kernels.hip
 constexpr uint32_t kBudgetMaxCount = 255;  // rounds a budget may assign

@@ -135,6 +135,7 @@ def load_library(path=LIB_PATH):
         "felog_budget_set": (C.c_int32, [P, C.POINTER(BudgetModel)]),
         "felog_budget_copy_out": (C.c_int32, [P, P, P, C.POINTER(C.c_uint32)]),
         "felog_run_budget": (C.c_int32, [P]),
+        "felog_budget_recheck": (C.c_int32, [P, C.c_int]),
         "felog_run_graph": (C.c_int32, [P, C.c_uint32]),
         "felog_corpus_get": (C.c_int32, [P, C.POINTER(Corpus)]),
         "felog_tile_shape_set": (C.c_int32, [P, C.POINTER(TileShape)]),
@@ -287,6 +288,11 @@ class Felog:
 
     def run_budget(self):
         self._check(self.lib.felog_run_budget(self.ctx), "felog_run_budget")
+
+    def budget_recheck(self, on=True):
+        """Score the budgeted vertices whose count ends at min_R once there;
+        those not converged walk one more round."""
+        self._check(self.lib.felog_budget_recheck(self.ctx, int(on)), "felog_budget_recheck")
 
     def run_graph(self, max_rounds=0):
         """run() / run_budget() with each round replayed from a hipGraph;

@@ -151,6 +151,41 @@ def test_budget_rounds_replay(rmat):
         fe.close()
 
 
+def test_budget_recheck_replays():
+    """The budget recheck on race-free tiles (every vertex budgeted: count 2,
+    full-length walks, min_R 2): the captured run and the eager
+    felog_run_budget extend the same counts to 3, walk the same third round
+    and train the same tables; flags are left alone."""
+    *g, n = cycles()
+    kw = dict(min_R=2, **RACE_FREE)
+    fg = ctx(*g, n, **kw)
+    fe = ctx(*g, n, **kw)
+    try:
+        m = walk_budget.constant_model(2, 22)
+        for f in (fg, fe):
+            f.budget_set(m)
+            f.budget_recheck(True)
+            f.visits_reset()
+        fg.run_graph()
+        fe.run_budget()
+        cg, _, _ = fg.budget()
+        ce, _, _ = fe.budget()
+        walked = np.arange(RACE_FREE["walkers"])  # walker i starts at vertex i
+        print(f"recheck: {int((cg == 3).sum())} of {n} counts extended, "
+              f"{int((cg[walked] == 3).sum())} of the walked vertices; rounds {fg.stats()['rounds_done']}")
+        assert np.array_equal(cg, ce)
+        assert set(np.unique(cg).tolist()) <= {2, 3}
+        want = 3 if (ce[walked] == 3).any() else 2
+        assert fg.stats()["rounds_done"] == fe.stats()["rounds_done"] == want
+        vg, ve = fg.visit_stats(), fe.visit_stats()
+        assert vg["visits"] == ve["visits"]
+        assert np.array_equal(fg.embeddings()[0], fe.embeddings()[0])
+        assert not fg.flags()[0].any() and not fe.flags()[0].any()
+    finally:
+        fg.close()
+        fe.close()
+
+
 def test_bf16_replays(rmat):
     """BF16 tables: two replays walk the eager corpora and train tables of
     the same scale."""
