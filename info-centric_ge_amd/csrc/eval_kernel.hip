@@ -28,9 +28,34 @@ __device__ __forceinline__ float ld<float>(const float *p) { return *p; }
 template <>
 __device__ __forceinline__ float ld<uint16_t>(const uint16_t *p) { return bf16_to_f32(*p); }
 
-__device__ __forceinline__ float wave_sum(float x) {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+// sum over the 16 lanes of this lane's group
+__device__ __forceinline__ float group_sum16(float x) {
+    for (int o = 8; o > 0; o >>= 1) x += __shfl_xor(x, o);
     return x;
+}
+
+// Columns e0 .. e0+7 of a row (0 past d); row pitches are multiples of 32
+// elements and e0 of 8, so whole groups are one or two aligned 16-byte loads.
+template <typename T>
+__device__ __forceinline__ void load8(const T *row, uint32_t e0, uint32_t d, float (&o)[8]) {
+    if (e0 + 8 <= d) {
+        if constexpr (sizeof(T) == 4) {
+            const float4 a = *(const float4 *)(row + e0), b = *(const float4 *)(row + e0 + 4);
+            o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+            o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+        } else {
+            const uint4 a = *(const uint4 *)(row + e0);
+            const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                o[2 * j] = bf16_to_f32((uint16_t)(w[j] & 0xFFFFu));
+                o[2 * j + 1] = bf16_to_f32((uint16_t)(w[j] >> 16));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) o[j] = e0 + j < d ? ld(row + e0 + j) : 0.f;
+    }
 }
 
 template <typename T>
@@ -75,29 +100,44 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                     if (lane == i) mine = dup ? j : t;
                 }
             }
+            // Four neighbours at a time, one per 16-lane group: lane l of a
+            // group holds columns 8l .. 8l+7 of every 128-column block and
+            // the group reduces the dot product and the norm over 16 lanes.
+            const uint32_t grp = lane >> 4, l = lane & 15;
+            const T *rv = syn0 + (uint64_t)v * dp;
             float nn = 0.f;
-            for (uint32_t e = lane; e < d; e += 64) {
-                const float x = ld(syn0 + (uint64_t)v * dp + e);
-                nn += x * x;
+            for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
+                float x[8];
+                load8(rv, e0, d, x);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) nn += x[j] * x[j];
             }
-            nn = wave_sum(nn);
+            nn = group_sum16(nn);
             const float nv = sqrtf(nn);
-            float sum = 0.f;
-            for (uint32_t i = 0; i < cnt; ++i) {
-                const uint32_t idx = __shfl(mine, (int)i);
-                const uint32_t u = col[b + idx];
+            float sum = 0.f;  // this group's neighbours
+            for (uint32_t i0 = 0; i0 < cnt; i0 += 4) {
+                const uint32_t i = i0 + grp;
+                const uint32_t idx = __shfl(mine, (int)(i & 63));
+                const bool have = i < cnt;
+                const T *ru = syn0 + (uint64_t)(have ? col[b + idx] : v) * dp;
                 float dot = 0.f, mm = 0.f;
-                for (uint32_t e = lane; e < d; e += 64) {
-                    const float x = ld(syn0 + (uint64_t)v * dp + e);
-                    const float y = ld(syn0 + (uint64_t)u * dp + e);
-                    dot += x * y;
-                    mm += y * y;
+                for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
+                    float x[8], y[8];
+                    load8(rv, e0, d, x);
+                    load8(ru, e0, d, y);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        dot += x[j] * y[j];
+                        mm += y[j] * y[j];
+                    }
                 }
-                dot = wave_sum(dot);
-                mm = wave_sum(mm);
+                dot = group_sum16(dot);
+                mm = group_sum16(mm);
                 const float nu = sqrtf(mm);
-                sum += (nv == 0.f || nu == 0.f) ? 0.f : dot / (nv * nu);
+                if (have) sum += (nv == 0.f || nu == 0.f) ? 0.f : dot / (nv * nu);
             }
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
             score = sum / (float)cnt;
         }
         if (lane == 0) {
