@@ -252,6 +252,8 @@ def main():
 
     for r in range(1, args.warmup + 1):
         one_round(r)
+    if args.warmup == 0 and not args.no_tune:
+        f.walk_round(1)  # a corpus for the autotuner; round 1 is walked again when timed
     # the autotuner times every tile shape on the last warm-up corpus (tables
     # restored after each pass) and installs the fastest
     default_shape = f.tile_shape()
