@@ -79,8 +79,8 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long n_eval = 0, n_conv = 0;
     for (uint32_t v = wave; v < V; v += nwaves) {
-        // vertices with a learned walk budget are not judged by convergence,
-        // except by the recheck when their count ends this round
+        // vertices with a learned walk budget take no convergence test,
+        // except the recheck's when their count ends this round
         if (!in_vocab[v] || flags[v]) continue;
         const uint32_t bc = budget_cnt ? budget_cnt[v] : 0u;
         if (recheck ? bc != round : bc != 0u) continue;

@@ -1,7 +1,8 @@
 """FP64 restatement of the learned walk budget and the visit diagnostics.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, always as the checker, never as the product.
+CPU-baseline leg of bench.py, always as the reference results are compared
+against, never as the product.
 
 The network is the library's 8-64-64-2 ReLU MLP with the same flat parameter
 layout (row-major, out x in): W1 (64x8), b1, W2 (64x64), b2, W3 (2x64), b3.

@@ -1,7 +1,8 @@
 """ctypes binding of liboracle.so - the CPU restatement of the walk -> SGNS path.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, always as the checker, never as the product.
+CPU-baseline leg of bench.py, always as the reference results are compared
+against, never as the product.
 """
 from __future__ import annotations
 
