@@ -289,8 +289,9 @@ felog_status felog_budget_train(felog_ctx *ctx, const float *X, const float *T,
  * vertices (no entropy state) and leave the rest to the plain rule.
  * model NULL removes the budget. */
 felog_status felog_budget_set(felog_ctx *ctx, const felog_budget_model *model);
-/* Predicted budgets (0 = plain rule); either pointer may be NULL.  *rounds
- * receives the largest count. */
+/* Current budgets (0 = plain rule); any pointer may be NULL.  The counts
+ * are the predicted ones as raised by the min_R recheck, if it ran;
+ * *rounds receives the largest current count. */
 felog_status felog_budget_copy_out(felog_ctx *ctx, uint16_t *count,
                                    uint16_t *length, uint32_t *rounds);
 /* Rounds of the budgeted run: walks and training until every budgeted

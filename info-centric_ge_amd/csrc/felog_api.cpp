@@ -631,9 +631,20 @@ felog_status felog_budget_copy_out(felog_ctx *c, uint16_t *count, uint16_t *leng
     felog_status s = need_built(c);
     if (s) return s;
     if (!c->budget_on) return fail(c, FELOG_ERR_STATE, "no budget installed");
-    if (count) TRY(c, hipMemcpy(count, c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
     if (length) TRY(c, hipMemcpy(length, c->d_budget_len, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
-    if (rounds) *rounds = c->budget_rounds;
+    if (count || rounds) {
+        // the recheck may have raised counts since the model was installed
+        std::vector<uint16_t> tmp;
+        uint16_t *cnt = count;
+        if (!cnt) {
+            tmp.resize(c->V);
+            cnt = tmp.data();
+        }
+        TRY(c, hipMemcpy(cnt, c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost));
+        uint32_t mx = 0;
+        for (uint32_t v = 0; v < c->V; ++v) mx = cnt[v] > mx ? cnt[v] : mx;
+        if (rounds) *rounds = mx;
+    }
     return FELOG_OK;
 }
 

@@ -6,6 +6,8 @@ gets degree >= 1 because the reference asserts it, include/walk.hpp:778):
   rmat(n, m, a, b, c)        recursive-matrix edges, ids >= n rejected
   chung_lu(n, m, gamma)      endpoints drawn with probability ~ expected
                              degree, expected degrees ~ i^(-1/(gamma-1))
+  rmat_device, chung_lu_device   the same laws generated on the GPU (torch)
+                             for graphs of tens of millions of edges
 Weights: "uniform" -> U(0,1], "lognormal" -> LogNormal(0, 1).
 
 Binary graph file (include/storage.hpp:40-61,107-115; src/tools/gconverter.cpp:69-76):
@@ -17,6 +19,8 @@ edges is held out as positives, each only while both endpoints keep degree
 edges (and not repeated) are drawn as negatives.
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -116,31 +120,17 @@ def rmat(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform"):
     return _finish(_fit_patches(keys, n, m), n, rng, weights)
 
 
-def rmat_device(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform", chunk=1 << 27, log=None):
-    """R-MAT on the GPU (torch) for graphs whose host generation would take
-    minutes: the same quadrant law as rmat(), self loops and ids >= n
-    rejected, duplicates removed, a random m of the distinct edges kept,
-    isolated vertices patched, random orientation.  Edge order is random
-    (not generation order).  Returns host numpy arrays."""
+def _device_edges(n, m, g, seed, weights, draw, log=None):
+    """Distinct undirected edges from `draw(count)` (device endpoint pairs,
+    self loops and ids >= n rejected) until m exist; a random m of them
+    kept, isolated vertices patched, random orientation.  Host arrays."""
     import torch
     dev = torch.device("cuda")
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    scale = int(np.ceil(np.log2(max(n, 2))))
     keys = torch.empty(0, dtype=torch.int64, device=dev)
     while keys.numel() < m:
         need = int((m - keys.numel()) * 1.4) + 1024
         parts = [keys]
-        for lo in range(0, need, chunk):
-            cnt = min(chunk, need - lo)
-            src = torch.zeros(cnt, dtype=torch.int64, device=dev)
-            dst = torch.zeros(cnt, dtype=torch.int64, device=dev)
-            for _ in range(scale):
-                r = torch.rand(cnt, generator=g, device=dev, dtype=torch.float64)
-                sbit = r >= a + b
-                dbit = ((r >= a) & (r < a + b)) | (r >= a + b + c)
-                src = (src << 1) | sbit.to(torch.int64)
-                dst = (dst << 1) | dbit.to(torch.int64)
+        for src, dst in draw(need):
             ok = (src < n) & (dst < n) & (src != dst)
             lo_, hi_ = torch.minimum(src, dst)[ok], torch.maximum(src, dst)[ok]
             parts.append(lo_ * n + hi_)
@@ -148,7 +138,7 @@ def rmat_device(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform", chunk=1
         keys = torch.unique(torch.cat(parts))
         del parts
         if log:
-            log(f"rmat_device: {keys.numel()} distinct edges of {m}")
+            log(f"device generator: {keys.numel()} distinct edges of {m}")
     keys = keys[torch.randperm(keys.numel(), generator=g, device=dev)[:m]]
     src, dst = keys // n, keys % n
     del keys
@@ -166,6 +156,59 @@ def rmat_device(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform", chunk=1
     torch.cuda.empty_cache()
     rng = np.random.default_rng(seed)
     return out_s, out_d, _weights(rng, len(out_s), weights)
+
+
+def rmat_device(n, m, a=0.57, b=0.19, c=0.19, seed=1, weights="uniform", chunk=1 << 27, log=None):
+    """R-MAT on the GPU (torch) for graphs whose host generation would take
+    minutes: the same quadrant law as rmat(), self loops and ids >= n
+    rejected, duplicates removed, a random m of the distinct edges kept,
+    isolated vertices patched, random orientation.  Edge order is random
+    (not generation order).  Returns host numpy arrays."""
+    import torch
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    scale = int(np.ceil(np.log2(max(n, 2))))
+
+    def draw(need):
+        for lo in range(0, need, chunk):
+            cnt = min(chunk, need - lo)
+            src = torch.zeros(cnt, dtype=torch.int64, device=dev)
+            dst = torch.zeros(cnt, dtype=torch.int64, device=dev)
+            for _ in range(scale):
+                r = torch.rand(cnt, generator=g, device=dev, dtype=torch.float64)
+                sbit = r >= a + b
+                dbit = ((r >= a) & (r < a + b)) | (r >= a + b + c)
+                src = (src << 1) | sbit.to(torch.int64)
+                dst = (dst << 1) | dbit.to(torch.int64)
+            yield src, dst
+
+    return _device_edges(n, m, g, seed, weights, draw, log)
+
+
+def chung_lu_device(n, m, gamma=2.2, seed=1, weights="uniform", chunk=1 << 27, log=None):
+    """Chung-Lu on the GPU (torch): the law of chung_lu() (endpoints drawn
+    in proportion to expected degrees i^(-1/(gamma-1)), hubs at random ids),
+    with rmat_device()'s dedupe, subset, patching and orientation."""
+    import torch
+    dev = torch.device("cuda")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    ew = torch.arange(1, n + 1, dtype=torch.float64, device=dev).pow(-1.0 / (gamma - 1.0))
+    cdf = torch.cumsum(ew / ew.sum(), 0)
+    cdf[-1] = 1.0
+    perm = torch.randperm(n, generator=g, device=dev)
+
+    def draw(need):
+        for lo in range(0, need, chunk):
+            cnt = min(chunk, need - lo)
+            ends = []
+            for _ in range(2):
+                r = torch.rand(cnt, generator=g, device=dev, dtype=torch.float64)
+                ends.append(perm[torch.searchsorted(cdf, r, right=True).clamp(0, n - 1)])
+            yield ends[0], ends[1]
+
+    return _device_edges(n, m, g, seed, weights, draw, log)
 
 
 def chung_lu(n, m, gamma=2.2, seed=1, weights="uniform"):
@@ -231,3 +274,27 @@ def lp_split(src, dst, w, n, frac=0.1, seed=7):
         neg.append((u, v))
     keep = ~held
     return src[keep], dst[keep], w[keep], pos.astype(np.uint32), np.asarray(neg, np.uint32).reshape(-1, 2)
+
+
+def device_graph_file(gen, n, m, seed, weights, path, timeout=900):
+    """Generate with a device generator ("rmat_device" or "chung_lu_device")
+    in a child process and write the binary edge file to `path`.  The child
+    owns the GPU only while it generates, so a process that already drives
+    the GPU through libfelog_hip.so never loads a second HIP runtime."""
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    subprocess.run([sys.executable, os.path.join(here, "graphgen.py"), gen, str(n), str(m), str(seed), weights,
+                    path], check=True, timeout=timeout)
+    return path
+
+
+if __name__ == "__main__":
+    import sys
+    gen_name, n_, m_, seed_, law_, out_ = sys.argv[1:7]
+    t0 = __import__("time").perf_counter()
+    s_, d_, w_ = globals()[gen_name](int(n_), int(m_), seed=int(seed_), weights=law_,
+                                     log=lambda t: print(f"[graphgen] {t}", flush=True))
+    write_bin(out_, s_, d_, w_)
+    print(f"[graphgen] {gen_name} n={n_}: {len(s_)} edges written to {out_} in "
+          f"{__import__('time').perf_counter() - t0:.1f} s", flush=True)

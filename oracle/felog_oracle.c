@@ -75,38 +75,85 @@ int orc_build_csr(uint32_t V, uint64_t m, const uint32_t *src, const uint32_t *d
     return 0;
 }
 
-/* ---- Vose alias method (M. D. Vose, IEEE TSE 17(9), 1991) --------------
- * Pairing order: small and large work stacks, both last-in-first-out,
- * filled in column order; p_j = w_j * n / sum(w) in FP64 with the sum taken
- * in column order; after pairing small s with large l, p_l <- (p_l + p_s) - 1.
- * Column j keeps itself with probability thresh_j / 2^32. */
-static uint32_t quantise(double p) {
-    if (p >= 1.0) return 0xFFFFFFFFu;
-    if (!(p > 0.0)) return 0u;
-    return (uint32_t)(p * 4294967296.0);
+/* ---- alias tables: exact integer construction --------------------------
+ * The reference declares alias buckets (include/walk.hpp:87-103) but never
+ * builds them; this construction is the build's own.  For a vertex with
+ * n columns and weights w_j (negative weights count as 0):
+ *  1. S = the canonical FP64 sum of the weights: the columns are cut into
+ *     chunks of 4096; a chunk's sum is the xor-butterfly combination of 64
+ *     lane partials, lane l summing the chunk's entries j = l (mod 64) in
+ *     order; the chunk sums are combined by the same rule.
+ *  2. q_j = trunc(((w_j * n) / S) * 2^32) in FP64, an integer share of the
+ *     total n * 2^32; the remainder n * 2^32 - sum q goes to the largest
+ *     q_j (lowest column on ties).  S = 0 gives q_j = 2^32 for every j.
+ *  3. The sweep of Hubschle-Schneider and Sanders ("Parallel weighted
+ *     random sampling", ESA 2019) over integers: light columns (q < 2^32)
+ *     in column order take the current heavy column (q >= 2^32) as their
+ *     alias and its residual drops by 2^32 - q_light; a heavy column whose
+ *     residual falls below 2^32 becomes a bucket of that residual aliased
+ *     to the next heavy column, which absorbs the difference; heavy
+ *     columns never reduced keep their whole bucket.
+ * Every bucket then holds exactly 2^32 units, so P(j) = q_j / (n 2^32):
+ * the only error against w_j / S is the truncation.  Column j keeps itself
+ * with probability thresh_j / 2^32 (0xFFFFFFFF = a whole bucket). */
+#define ALIAS_ONE 4294967296ull
+
+static double lanesum(const double *x, uint64_t n) {
+    double p[64], q[64];
+    for (int l = 0; l < 64; ++l) p[l] = 0.0;
+    for (uint64_t j = 0; j < n; ++j) p[j & 63] = p[j & 63] + x[j];
+    for (int o = 32; o > 0; o >>= 1) {
+        for (int l = 0; l < 64; ++l) q[l] = p[l] + p[l ^ o];
+        memcpy(p, q, sizeof p);
+    }
+    return p[0];
 }
 
-static void vose(uint32_t n, const double *wts, double *p, uint32_t *small, uint32_t *large, uint32_t *thresh,
-                 uint32_t *alias_idx) {
-    double sum = 0.0;
-    for (uint32_t j = 0; j < n; ++j) sum = sum + (wts[j] > 0.0 ? wts[j] : 0.0);
-    uint32_t ns = 0, nl = 0;
-    for (uint32_t j = 0; j < n; ++j) {
-        p[j] = sum > 0.0 ? ((wts[j] > 0.0 ? wts[j] : 0.0) * (double)n) / sum : 1.0;
-        if (p[j] < 1.0) small[ns++] = j;
-        else large[nl++] = j;
+static double canonical_sum(const double *x, uint64_t n) {
+    const uint64_t nc = (n + 4095) / 4096;
+    double *cs = (double *)malloc((nc ? nc : 1) * sizeof(double));
+    for (uint64_t k = 0; k < nc; ++k) cs[k] = lanesum(x + 4096 * k, n - 4096 * k < 4096 ? n - 4096 * k : 4096);
+    const double s = lanesum(cs, nc);
+    free(cs);
+    return s;
+}
+
+/* x: n clamped weights; q, lt, hv: scratch of n entries. */
+static void alias_exact(uint64_t n, const double *x, uint64_t *q, uint32_t *lt, uint32_t *hv, uint32_t *thresh,
+                        uint32_t *alias_idx) {
+    if (n == 0) return;
+    const double S = canonical_sum(x, n);
+    if (S > 0.0) {
+        const double dn = (double)n;
+        uint64_t sq = 0, jmax = 0;
+        for (uint64_t j = 0; j < n; ++j) {
+            q[j] = (uint64_t)(((x[j] * dn) / S) * 4294967296.0);
+            sq += q[j];
+            if (q[j] > q[jmax]) jmax = j;
+        }
+        q[jmax] += n * ALIAS_ONE - sq;  /* exact: the true result is in range */
+    } else {
+        for (uint64_t j = 0; j < n; ++j) q[j] = ALIAS_ONE;
     }
-    while (ns > 0 && nl > 0) {
-        const uint32_t s = small[--ns];
-        const uint32_t l = large[--nl];
-        thresh[s] = quantise(p[s]);
-        alias_idx[s] = l;
-        p[l] = (p[l] + p[s]) - 1.0;
-        if (p[l] < 1.0) small[ns++] = l;
-        else large[nl++] = l;
+    uint64_t nl = 0, nh = 0;
+    for (uint64_t j = 0; j < n; ++j) {
+        if (q[j] < ALIAS_ONE) lt[nl++] = (uint32_t)j;
+        else hv[nh++] = (uint32_t)j;
     }
-    while (nl > 0) { const uint32_t l = large[--nl]; thresh[l] = 0xFFFFFFFFu; alias_idx[l] = l; }
-    while (ns > 0) { const uint32_t s = small[--ns]; thresh[s] = 0xFFFFFFFFu; alias_idx[s] = s; }
+    for (uint64_t j = 0; j < n; ++j) { thresh[j] = 0xFFFFFFFFu; alias_idx[j] = (uint32_t)j; }
+    uint64_t h = 0, r = nh ? q[hv[0]] : 0;
+    for (uint64_t i = 0; i < nl; ++i) {
+        const uint32_t l = lt[i];
+        thresh[l] = (uint32_t)q[l];
+        alias_idx[l] = hv[h];
+        r -= ALIAS_ONE - q[l];
+        while (r < ALIAS_ONE && h + 1 < nh) {
+            thresh[hv[h]] = (uint32_t)r;
+            alias_idx[hv[h]] = hv[h + 1];
+            r = q[hv[h + 1]] - (ALIAS_ONE - r);
+            ++h;
+        }
+    }
 }
 
 void orc_build_alias(uint32_t V, const uint64_t *row_ptr, const float *w, uint32_t *thresh, uint32_t *alias_idx) {
@@ -114,31 +161,39 @@ void orc_build_alias(uint32_t V, const uint64_t *row_ptr, const float *w, uint32
     for (uint32_t v = 0; v < V; ++v)
         if (row_ptr[v + 1] - row_ptr[v] > maxd) maxd = row_ptr[v + 1] - row_ptr[v];
     double *wd = (double *)malloc((maxd + 1) * sizeof(double));
-    double *p = (double *)malloc((maxd + 1) * sizeof(double));
-    uint32_t *sm = (uint32_t *)malloc((maxd + 1) * sizeof(uint32_t));
-    uint32_t *lg = (uint32_t *)malloc((maxd + 1) * sizeof(uint32_t));
+    uint64_t *q = (uint64_t *)malloc((maxd + 1) * sizeof(uint64_t));
+    uint32_t *lt = (uint32_t *)malloc((maxd + 1) * sizeof(uint32_t));
+    uint32_t *hv = (uint32_t *)malloc((maxd + 1) * sizeof(uint32_t));
     for (uint32_t v = 0; v < V; ++v) {
         const uint64_t b = row_ptr[v];
-        const uint32_t n = (uint32_t)(row_ptr[v + 1] - b);
-        for (uint32_t j = 0; j < n; ++j) wd[j] = w[b + j] > 0.f ? (double)w[b + j] : 0.0;
-        vose(n, wd, p, sm, lg, thresh + b, alias_idx + b);
+        const uint64_t n = row_ptr[v + 1] - b;
+        for (uint64_t j = 0; j < n; ++j) wd[j] = w[b + j] > 0.f ? (double)w[b + j] : 0.0;
+        alias_exact(n, wd, q, lt, hv, thresh + b, alias_idx + b);
     }
-    free(wd); free(p); free(sm); free(lg);
+    free(wd); free(q); free(lt); free(hv);
 }
 
 void orc_alias_1d(uint32_t n, const double *wts, uint32_t *thresh, uint32_t *alias_idx) {
-    double *p = (double *)malloc(((uint64_t)n + 1) * sizeof(double));
-    uint32_t *sm = (uint32_t *)malloc(((uint64_t)n + 1) * sizeof(uint32_t));
-    uint32_t *lg = (uint32_t *)malloc(((uint64_t)n + 1) * sizeof(uint32_t));
-    vose(n, wts, p, sm, lg, thresh, alias_idx);
-    free(p); free(sm); free(lg);
+    double *x = (double *)malloc(((uint64_t)n + 1) * sizeof(double));
+    uint64_t *q = (uint64_t *)malloc(((uint64_t)n + 1) * sizeof(uint64_t));
+    uint32_t *lt = (uint32_t *)malloc(((uint64_t)n + 1) * sizeof(uint32_t));
+    uint32_t *hv = (uint32_t *)malloc(((uint64_t)n + 1) * sizeof(uint32_t));
+    for (uint32_t j = 0; j < n; ++j) x[j] = wts[j] > 0.0 ? wts[j] : 0.0;
+    alias_exact(n, x, q, lt, hv, thresh, alias_idx);
+    free(x); free(q); free(lt); free(hv);
 }
 
 /* Negative sampler: weight deg^0.75 for vocabulary vertices, 0 otherwise
- * (src/word2vec.cu:674-694), as an alias table over vertex ids. */
+ * (src/word2vec.cu:674-694 draws from the same law through a table), as an
+ * alias table over vertex ids.  deg^0.75 = sqrt(deg) * sqrt(sqrt(deg)):
+ * correctly rounded square roots give the same bits on every IEEE machine,
+ * where pow() may differ by an ulp between libraries. */
 void orc_unigram(uint32_t V, const uint32_t *deg, const uint8_t *in_vocab, uint32_t *thresh, uint32_t *alias_idx) {
     double *wts = (double *)malloc(((uint64_t)V + 1) * sizeof(double));
-    for (uint32_t v = 0; v < V; ++v) wts[v] = in_vocab[v] ? pow((double)deg[v], 0.75) : 0.0;
+    for (uint32_t v = 0; v < V; ++v) {
+        const double x = (double)deg[v];
+        wts[v] = in_vocab[v] ? sqrt(x) * sqrt(sqrt(x)) : 0.0;
+    }
     orc_alias_1d(V, wts, thresh, alias_idx);
     free(wts);
 }
@@ -594,6 +649,50 @@ uint32_t orc_sgns_groups(const orc_sgns_params *p, const uint64_t *offsets, cons
 #define MAX_EXP 6
 #define MAX_SENTENCE 15000
 
+/* InitUnigramTable, src/word2vec.cu:674-694: slot a holds vocabulary index
+ * i while a / table_size has not passed the cumulative share of indices
+ * 0..i under cn^0.75; cn[] is in vocabulary (rank) order. */
+void orc_w2v_unigram_table(uint64_t vocab_size, const uint32_t *cn, uint32_t table_size, int32_t *table) {
+    double pw = 0.0;
+    for (uint64_t a = 0; a < vocab_size; ++a) pw += pow((double)cn[a], 0.75);
+    uint64_t i = 0;
+    double d1 = pow((double)cn[0], 0.75) / pw;
+    for (uint64_t a = 0; a < table_size; ++a) {
+        table[a] = (int32_t)i;
+        if (a / (double)table_size > d1) {
+            ++i;
+            if (i < vocab_size) d1 += pow((double)cn[i], 0.75) / pw;
+        }
+        if (i >= vocab_size) i = vocab_size - 1;
+    }
+}
+
+/* expTable, src/word2vec.cu:2292-2297: entry i = sigma((i / 1000 * 2 - 1) * 6),
+ * plus a copy of the last entry at index 1000. */
+void orc_w2v_exp_table(float *expTable) {
+    for (int i = 0; i < EXP_TABLE_SIZE; ++i) {
+        expTable[i] = expf((i / (float)EXP_TABLE_SIZE * 2 - 1) * MAX_EXP);
+        expTable[i] = expTable[i] / (expTable[i] + 1);
+    }
+    expTable[EXP_TABLE_SIZE] = expTable[EXP_TABLE_SIZE - 1];
+}
+
+/* Index of the expTable entry for a logit f in (-6, 6]: the integer
+ * division EXP_TABLE_SIZE / MAX_EXP / 2 is 83, not 83.33
+ * (src/word2vec.cu:525). */
+int orc_w2v_lut_index(float f) { return (int)((f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2)); }
+
+/* One negative draw of skip_gram_kernel (src/word2vec.cu:500-505): advance
+ * the LCG, take table[(x >> 16) % table_size]; index 0 is replaced by
+ * x % (vocab_size - 1) + 1.  Returns the vocabulary index. */
+uint32_t orc_w2v_negative(uint64_t *next_random, const int32_t *table, uint32_t table_size, uint64_t vocab_size) {
+    const unsigned long long x = (unsigned long long)*next_random * 25214903917ull + 11ull;
+    *next_random = x;
+    uint32_t tr = (uint32_t)table[(x >> 16) % table_size];
+    if (tr == 0) tr = (uint32_t)(x % (vocab_size - 1) + 1);
+    return tr;
+}
+
 uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const uint64_t *offsets, const uint32_t *tokens,
                        uint64_t n_sent, const uint32_t *deg, const uint32_t *order, const uint32_t *rank,
                        const uint8_t *in_vocab, uint64_t vocab_size, uint64_t train_words, uint32_t window,
@@ -601,26 +700,14 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                        uint32_t table_size, int threads, uint64_t *pairs_out) {
     /* expTable, src/word2vec.cu:2292-2297 */
     float expTable[EXP_TABLE_SIZE + 1];
-    for (int i = 0; i < EXP_TABLE_SIZE; ++i) {
-        expTable[i] = expf((i / (float)EXP_TABLE_SIZE * 2 - 1) * MAX_EXP);
-        expTable[i] = expTable[i] / (expTable[i] + 1);
-    }
-    expTable[EXP_TABLE_SIZE] = expTable[EXP_TABLE_SIZE - 1];
+    orc_w2v_exp_table(expTable);
     /* unigram table over vocabulary ranks, src/word2vec.cu:674-694 */
-    int *table = (int *)malloc((uint64_t)table_size * sizeof(int));
+    int32_t *table = (int32_t *)malloc((uint64_t)table_size * sizeof(int32_t));
     {
-        double pw = 0.0;
-        for (uint64_t a = 0; a < vocab_size; ++a) pw += pow((double)deg[order[a]], 0.75);
-        uint64_t i = 0;
-        double d1 = pow((double)deg[order[0]], 0.75) / pw;
-        for (uint64_t a = 0; a < table_size; ++a) {
-            table[a] = (int)i;
-            if (a / (double)table_size > d1) {
-                ++i;
-                if (i < vocab_size) d1 += pow((double)deg[order[i]], 0.75) / pw;
-            }
-            if (i >= vocab_size) i = vocab_size - 1;
-        }
+        uint32_t *cn = (uint32_t *)malloc((vocab_size ? vocab_size : 1) * sizeof(uint32_t));
+        for (uint64_t a = 0; a < vocab_size; ++a) cn[a] = deg[order[a]];
+        orc_w2v_unigram_table(vocab_size, cn, table_size, table);
+        free(cn);
     }
     /* subsampling thresholds (u16) and the batching RNG, src/word2vec.cu:134-152,180-212 */
     uint16_t *thr16 = (uint16_t *)malloc((uint64_t)V * sizeof(uint16_t));
@@ -633,13 +720,20 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
         }
     }
     unsigned long long lcg = 0x9E3779B97F4A7C15ull ^ (1ull << 1) ^ 0x2ull;
-    /* serial batching pass: filtered sentences, capped at 1000 tokens */
+    /* serial batching pass (src/word2vec.cu:1194-1248): filtered sentences,
+     * capped at 1000 tokens; a batch closes at 15000 sentences or 299,980
+     * tokens, and then draws its per-sentence negatives from the same LCG
+     * (:1251-1259) whether or not they are used.  blk[s] = the sentence's
+     * index in its batch, the kernel's blockIdx.x. */
     uint64_t nt = offsets[n_sent];
     uint32_t *sen = (uint32_t *)malloc((nt ? nt : 1) * sizeof(uint32_t));
     uint64_t *soff = (uint64_t *)malloc((n_sent + 1) * sizeof(uint64_t));
-    uint64_t words = 0, total = 0;
+    uint32_t *blk = (uint32_t *)malloc((n_sent ? n_sent : 1) * sizeof(uint32_t));
+    uint64_t words = 0, total = 0, batch_start = 0;
+    uint32_t in_batch = 0;
     soff[0] = 0;
     for (uint64_t s = 0; s < n_sent; ++s) {
+        blk[s] = in_batch;
         uint32_t len = 0;
         for (uint64_t i = offsets[s]; i < offsets[s + 1]; ++i) {
             const uint32_t x = tokens[i];
@@ -658,6 +752,12 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
             if (++len >= 1000) break;
         }
         soff[s + 1] = total;
+        ++in_batch;
+        if (in_batch == MAX_SENTENCE || total - batch_start >= (uint64_t)(MAX_SENTENCE - 1) * 20 || s + 1 == n_sent) {
+            for (uint64_t q = 0; q < (uint64_t)in_batch * negative; ++q) lcg = lcg * 25214903917ull + 11ull;
+            in_batch = 0;
+            batch_start = total;
+        }
     }
     uint64_t pairs = 0;
 #ifdef _OPENMP
@@ -669,7 +769,7 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
 #pragma omp for schedule(dynamic, 64)
         for (int64_t s = 0; s < (int64_t)n_sent; ++s) {
             const uint64_t ss = soff[s], se = soff[s + 1];
-            unsigned long long next_random = (unsigned long long)(s % MAX_SENTENCE);
+            unsigned long long next_random = (unsigned long long)blk[s];
             for (uint64_t pos = ss; pos < se; ++pos) {
                 const uint32_t word = sen[pos];
                 next_random = next_random * 2514903917ull + 11;  /* src/word2vec.cu:458 (sic) */
@@ -689,9 +789,9 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                             target = word;
                             label = 1.f;
                         } else {
-                            next_random = next_random * 25214903917ull + 11;
-                            uint32_t tr = (uint32_t)table[(next_random >> 16) % table_size];
-                            if (tr == 0) tr = (uint32_t)(next_random % (vocab_size - 1) + 1);
+                            uint64_t nr = next_random;
+                            const uint32_t tr = orc_w2v_negative(&nr, table, table_size, vocab_size);
+                            next_random = nr;
                             target = order[tr];
                             if (target == word) continue;
                             label = 0.f;
@@ -703,7 +803,7 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                         float g;
                         if (f > MAX_EXP) g = (label - 1) * alpha;
                         else if (f < -MAX_EXP) g = (label - 0) * alpha;
-                        else if (use_lut) g = (label - expTable[(int)((f + MAX_EXP) * (EXP_TABLE_SIZE / MAX_EXP / 2))]) * alpha;
+                        else if (use_lut) g = (label - expTable[orc_w2v_lut_index(f)]) * alpha;
                         else g = (label - 1.f / (1.f + expf(-f))) * alpha;
                         for (uint32_t e = 0; e < d; ++e) neu1e[e] += g * l2[e];
                         for (uint32_t e = 0; e < d; ++e) l2[e] += g * l1[e];
@@ -715,11 +815,33 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
         free(neu1e);
     }
     if (pairs_out) *pairs_out = pairs;
-    free(table); free(thr16); free(sen); free(soff);
+    free(table); free(thr16); free(sen); free(soff); free(blk);
     return words;
 }
 
 /* ---- convergence evaluation ------------------------------------------ */
+/* Neighbour columns scored for vertex v (src/word2vec.cu:1785-1800 takes
+ * at most 30 random neighbours): all of them when deg <= 30, else 30
+ * distinct columns by Floyd's sampling (Bentley & Floyd, CACM 30(9),
+ * 1987): for j = deg-30 .. deg-1 draw t uniform in [0, j] (Philox word 0
+ * mod j+1) and take t, or j when t was already taken. */
+uint32_t orc_eval_subset(uint32_t deg, uint32_t v, uint32_t round, uint64_t seed, uint32_t *chosen) {
+    if (deg <= 30) {
+        for (uint32_t i = 0; i < deg; ++i) chosen[i] = i;
+        return deg;
+    }
+    for (uint32_t i = 0; i < 30; ++i) {
+        const uint32_t j = deg - 30 + i;
+        uint32_t r[4];
+        rnd(seed, v, D_EVAL | i, 0xFFFFFFFFu, round, r);
+        const uint32_t t = r[0] % (j + 1);
+        int dup = 0;
+        for (uint32_t q = 0; q < i; ++q) dup |= chosen[q] == t;
+        chosen[i] = dup ? j : t;
+    }
+    return 30;
+}
+
 uint32_t orc_eval(uint32_t V, uint32_t d, const float *syn0, const uint64_t *row_ptr, const uint32_t *col,
                   const uint8_t *in_vocab, uint8_t *flags, float *scores, uint32_t round, uint64_t seed) {
     uint32_t eva = 0;
@@ -730,21 +852,7 @@ uint32_t orc_eval(uint32_t V, uint32_t d, const float *syn0, const uint64_t *row
         float score = 0.f;
         if (deg > 0) {
             uint32_t chosen[30];
-            const uint32_t cnt = deg < 30 ? deg : 30;
-            if (deg <= 30) {
-                for (uint32_t i = 0; i < cnt; ++i) chosen[i] = i;
-            } else {
-                /* Floyd's sampling of 30 distinct indices (Bentley & Floyd, CACM 30(9), 1987) */
-                for (uint32_t i = 0; i < 30; ++i) {
-                    const uint32_t j = deg - 30 + i;
-                    uint32_t r[4];
-                    rnd(seed, v, D_EVAL | i, 0xFFFFFFFFu, round, r);
-                    const uint32_t t = r[0] % (j + 1);
-                    int dup = 0;
-                    for (uint32_t q = 0; q < i; ++q) dup |= chosen[q] == t;
-                    chosen[i] = dup ? j : t;
-                }
-            }
+            const uint32_t cnt = orc_eval_subset(deg, v, round, seed, chosen);
             const float *x = syn0 + (uint64_t)v * d;
             float nn = 0.f;
             for (uint32_t e = 0; e < d; ++e) nn += x[e] * x[e];

@@ -9,7 +9,8 @@
  * repository) or the published method it restates.  Pins (each in tests/):
  *   orc_philox4x32_10      Random123 known-answer vectors
  *   orc_build_csr          brute-force adjacency on small graphs; order rule
- *   orc_build_alias        exact probability identity vs w_j / sum(w)
+ *   orc_build_alias        exact probability identity vs w_j / sum(w);
+ *                          every bucket holds one unit; chi-square draws
  *   orc_corr_reference /
  *   orc_corr_exact         Shannon entropy and two-pass Pearson recomputed
  *                          from the visit histogram (closed forms)
@@ -19,8 +20,17 @@
  *   orc_tanh_restated      libm tanh within a few ulp
  *   orc_init_syn0          the reference's sequential LCG loop, element 0..n
  *   orc_sgns_groups        finite differences of the SGNS loss on d <= 8
- *   orc_w2v_train          single-pair update == hand-computed SGNS step
+ *   orc_w2v_train          single-pair update == hand-computed SGNS step;
+ *                          with negatives and the sigmoid table: an FP64
+ *                          restatement driven by the same LCG draws
+ *   orc_w2v_unigram_table  cumulative slot counts vs table_size * the
+ *                          cn^0.75 law (within one slot)
+ *   orc_w2v_lut_index /
+ *   orc_w2v_exp_table      hand-computed indices; entries vs the logistic
+ *   orc_w2v_negative       a hand-traced LCG step and table lookup
  *   orc_eval               brute-force cosine on small tables
+ *   orc_eval_subset        distinct in-range columns; inclusion frequency
+ *                          30/deg (chi-square over many draws)
  *   orc_auc                scikit-learn roc_auc_score (fixtures)
  */
 #ifndef FELOG_ORACLE_H
@@ -37,13 +47,14 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 int orc_build_csr(uint32_t V, uint64_t m, const uint32_t *src, const uint32_t *dst, const float *w,
                   int undirected, uint64_t *row_ptr, uint32_t *col, float *wout);
 
-/* Vose alias tables per vertex (FP64 normalisation, u32 thresholds). */
+/* Alias tables per vertex: canonical FP64 sum, integer shares of n * 2^32,
+ * integer sweep pairing (u32 thresholds, column indices). */
 void orc_build_alias(uint32_t V, const uint64_t *row_ptr, const float *w, uint32_t *thresh,
                      uint32_t *alias_idx);
 /* Alias over one weight vector. */
 void orc_alias_1d(uint32_t n, const double *wts, uint32_t *thresh, uint32_t *alias_idx);
-/* Negative sampler over vertex ids: deg^0.75 (libm pow) for vocabulary
- * vertices, 0 otherwise. */
+/* Negative sampler over vertex ids: deg^0.75 (as sqrt(deg) * sqrt(sqrt(deg)))
+ * for vocabulary vertices, 0 otherwise. */
 void orc_unigram(uint32_t V, const uint32_t *deg, const uint8_t *in_vocab, uint32_t *thresh, uint32_t *alias_idx);
 
 /* n*log2(n) for n = 0..L+1 and its forward differences for f = 0..L. */
@@ -110,6 +121,22 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                        const uint32_t *rank, const uint8_t *in_vocab, uint64_t vocab_size,
                        uint64_t train_words, uint32_t window, uint32_t negative, float sample, float alpha,
                        int use_lut, int compat_break, uint32_t table_size, int threads, uint64_t *pairs);
+
+/* word2vec's unigram table over vocabulary indices (src/word2vec.cu:674-694);
+ * cn[] = counts in vocabulary order. */
+void orc_w2v_unigram_table(uint64_t vocab_size, const uint32_t *cn, uint32_t table_size, int32_t *table);
+/* expTable[0..1000] (src/word2vec.cu:2292-2297) and the index a logit
+ * f in (-6, 6] reads (src/word2vec.cu:525). */
+void orc_w2v_exp_table(float *expTable);
+int orc_w2v_lut_index(float f);
+/* One negative draw of skip_gram_kernel (src/word2vec.cu:500-505); advances
+ * *next_random and returns a vocabulary index. */
+uint32_t orc_w2v_negative(uint64_t *next_random, const int32_t *table, uint32_t table_size, uint64_t vocab_size);
+
+/* Neighbour columns the evaluation scores for vertex v of degree deg:
+ * 0..deg-1 when deg <= 30, else 30 distinct columns by Floyd's sampling.
+ * Returns the count written to chosen[30]. */
+uint32_t orc_eval_subset(uint32_t deg, uint32_t v, uint32_t round, uint64_t seed, uint32_t *chosen);
 
 /* Convergence scores (src/word2vec.cu:1748-1889,2016-2080) for every vertex
  * with in_vocab && !flags; others get score 0.  Returns #not converged and

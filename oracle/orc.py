@@ -245,3 +245,39 @@ def auc(pos, neg):
     pos = np.ascontiguousarray(pos, np.float32)
     neg = np.ascontiguousarray(neg, np.float32)
     return float(lib().orc_auc(C.c_uint64(len(pos)), P(pos), C.c_uint64(len(neg)), P(neg)))
+
+
+def w2v_unigram_table(cn_by_rank, table_size):
+    """word2vec's unigram table (vocabulary indices) over counts in rank order."""
+    cn = np.ascontiguousarray(cn_by_rank, np.uint32)
+    table = np.empty(table_size, np.int32)
+    lib().orc_w2v_unigram_table(C.c_uint64(len(cn)), P(cn), C.c_uint32(table_size), P(table))
+    return table
+
+
+def w2v_exp_table():
+    t = np.empty(1001, np.float32)
+    lib().orc_w2v_exp_table(P(t))
+    return t
+
+
+def w2v_lut_index(f):
+    lib().orc_w2v_lut_index.argtypes = [C.c_float]
+    return int(lib().orc_w2v_lut_index(C.c_float(f)))
+
+
+def w2v_negative(next_random, table, vocab_size):
+    """One negative draw: returns (vocabulary index, advanced LCG state)."""
+    nr = C.c_uint64(next_random)
+    table = np.ascontiguousarray(table, np.int32)
+    lib().orc_w2v_negative.restype = C.c_uint32
+    r = lib().orc_w2v_negative(C.byref(nr), P(table), C.c_uint32(len(table)), C.c_uint64(vocab_size))
+    return int(r), int(nr.value)
+
+
+def eval_subset(deg, v, rnd, seed):
+    """Neighbour columns the evaluation scores for vertex v of degree deg."""
+    out = np.empty(30, np.uint32)
+    lib().orc_eval_subset.restype = C.c_uint32
+    n = lib().orc_eval_subset(C.c_uint32(deg), C.c_uint32(v), C.c_uint32(rnd), C.c_uint64(seed), P(out))
+    return out[:n].copy()

@@ -76,6 +76,26 @@ def test_alias_identity(case):
     assert np.max(np.abs(p - target)) <= len(w) * 2.0 ** -32 + 1e-12
 
 
+@pytest.mark.parametrize("n", [3, 64, 4097, 20_000])
+def test_alias_integer_shares(n):
+    """Every column's probability is its integer share q_j / (n 2^32) with
+    q_j = floor(w_j n 2^32 / sum w): off w_j / sum w by less than one unit
+    of 2^-32 / n, except the largest column, which also takes the rounding
+    remainder (at most n units).  Rows longer than 4096 exercise the
+    chunked sum."""
+    rng = np.random.default_rng(n)
+    w = rng.lognormal(0, 2, n).astype(np.float32)
+    t, a = orc.build_alias(1, np.array([0, n], np.uint64), w)
+    p = alias_probs(t, a)
+    target = w.astype(np.float64) / w.astype(np.float64).sum()
+    unit = 2.0 ** -32 / n
+    jmax = int(np.argmax(target))
+    err = np.abs(p - target)
+    assert err[jmax] <= (n + 2) * unit
+    err[jmax] = 0
+    assert err.max() <= 1.01 * unit + 1e-14 * target.max()
+
+
 def test_unigram_law():
     rng = np.random.default_rng(6)
     deg = rng.integers(1, 300, 400).astype(np.uint32)
@@ -545,3 +565,173 @@ def test_visit_stats_closed_form():
     assert abs(kl - (0.5 * math.log(2) + 0.25 * math.log(1))) < 1e-15
     assert abs(ent - (2 * 0.25 * math.log(4) + 0.5 * math.log(2))) < 1e-15
     assert unv == 1
+
+
+# ---- evaluation subset: Floyd's sampling of 30 of deg > 30 neighbours
+@pytest.mark.parametrize("deg", [31, 45, 200, 5000])
+def test_eval_subset_is_uniform(deg):
+    """Each draw is 30 distinct columns in [0, deg); every column is included
+    with probability 30/deg (chi-square over 20,000 draws)."""
+    from scipy.stats import chi2
+    n = 20_000
+    counts = np.zeros(deg, np.int64)
+    for k in range(n):
+        s = orc.eval_subset(deg, k % 997, 1 + k // 997, 0xABCDEF)
+        assert len(s) == 30 and len(np.unique(s)) == 30 and s.max() < deg
+        counts[s] += 1
+    expect = n * 30 / deg
+    stat = ((counts - expect) ** 2 / expect).sum()
+    assert stat < chi2.ppf(0.9999, deg - 1), (stat, deg)
+
+
+def test_eval_subset_small_degree_takes_all_and_pairs_are_uniform():
+    assert np.array_equal(orc.eval_subset(7, 3, 2, 9), np.arange(7))
+    assert np.array_equal(orc.eval_subset(30, 3, 2, 9), np.arange(30))
+    # deg 32: a subset is fixed by its two missing columns; all 496 pairs
+    # must be equally likely
+    from scipy.stats import chi2
+    n = 49_600
+    hist = {}
+    for k in range(n):
+        miss = tuple(sorted(set(range(32)) - set(orc.eval_subset(32, k, 5, 77).tolist())))
+        hist[miss] = hist.get(miss, 0) + 1
+    assert len(hist) == 496
+    c = np.array(list(hist.values()))
+    stat = ((c - 100.0) ** 2 / 100.0).sum()
+    assert stat < chi2.ppf(0.9999, 495)
+
+
+def test_eval_brute_force_hubs():
+    """Vertices with more than 30 neighbours: the score is the mean cosine
+    over exactly the columns Floyd's sampling picks."""
+    rng = np.random.default_rng(8)
+    V, d = 600, 16
+    hubs = np.arange(6)
+    src = np.concatenate([np.repeat(hubs, 90), rng.integers(6, V, 400)]).astype(np.uint32)
+    dst = np.concatenate([rng.choice(np.arange(6, V), 540), rng.integers(6, V, 400)]).astype(np.uint32)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    rp, col, _ = orc.build_csr(V, src, dst, np.ones(len(src), np.float32), True)
+    deg = np.diff(rp.astype(np.int64))
+    assert (deg > 30).sum() >= 6
+    syn0 = (rng.standard_normal(d) + 0.8 * rng.standard_normal((V, d))).astype(np.float32)
+    eva, flags, scores = orc.evaluate(syn0, rp, col, np.ones(V, np.uint8), np.zeros(V, np.uint8), 4, 0x77)
+    for v in range(V):
+        if deg[v] == 0:
+            continue
+        cols = orc.eval_subset(int(deg[v]), v, 4, 0x77)
+        nb = col[rp[v] + cols.astype(np.uint64)]
+        x = syn0[v].astype(np.float64)
+        ref = np.mean([x @ syn0[u] / np.linalg.norm(x) / np.linalg.norm(syn0[u]) for u in nb])
+        assert abs(scores[v] - ref) < 1e-5
+        assert bool(flags[v]) == (scores[v] > 0.65)
+
+
+# ---- word2vec restatement: unigram table, sigmoid table, negative draws
+def test_w2v_unigram_table_law():
+    """Slot a holds index i until a / table_size passes the cumulative
+    share D_i of cn^0.75 (src/word2vec.cu:674-694), the slot that passes it
+    included, so indices 0..i hold floor(table_size * D_i) + 2 slots."""
+    rng = np.random.default_rng(4)
+    cn = np.sort(rng.zipf(1.8, 3000).clip(1, 50_000))[::-1].astype(np.uint32)
+    T = 2_000_000
+    table = orc.w2v_unigram_table(cn, T)
+    assert table.min() == 0 and table.max() == len(cn) - 1
+    assert np.all(np.diff(table) >= 0) and np.all(np.diff(table) <= 1)
+    counts = np.bincount(table, minlength=len(cn))
+    share = np.cumsum(cn.astype(np.float64) ** 0.75) / (cn.astype(np.float64) ** 0.75).sum()
+    gap = np.cumsum(counts)[:-1] - T * share[:-1]
+    assert gap.min() >= 1.0 - 1e-6 and gap.max() <= 2.0 + 1e-6
+
+
+def test_w2v_sigmoid_table_and_index():
+    t = orc.w2v_exp_table()
+    x = (np.arange(1000) / 1000.0 * 2 - 1) * 6
+    assert np.abs(t[:1000] - 1 / (1 + np.exp(-x))).max() < 2e-7
+    assert t[1000] == t[999]
+    # (int)((f + 6) * 83): the integer division 1000/6/2 is 83
+    for f, idx in ((0.0, 498), (1.0, 581), (-6.0, 0), (5.999, 995), (-0.5, 456), (2.5, 705), (-5.99, 0)):
+        assert orc.w2v_lut_index(f) == idx, f
+
+
+def test_w2v_negative_draw_hand_traced():
+    table = np.array([0, 0, 1, 1, 1, 2, 3, 3], np.int32)
+    vocab = 4
+    for seed in (0, 1, 12345, 2 ** 63 + 17, 987654321987):
+        x = (seed * 25214903917 + 11) % (1 << 64)
+        tr = int(table[(x >> 16) % 8])
+        if tr == 0:
+            tr = x % (vocab - 1) + 1
+        got, nxt = orc.w2v_negative(seed, table, vocab)
+        assert (got, nxt) == (tr, x)
+
+
+def _w2v_fp64(syn0, syn1, sent, order, window, negative, alpha, table, vocab_size):
+    """skip_gram_kernel (src/word2vec.cu:442-536) per pair, sequentially, in
+    FP64, with the sigmoid table and the kernel's two LCGs; sentences of
+    vertex ids, block index = sentence index."""
+    s0 = syn0.astype(np.float64)
+    s1 = syn1.astype(np.float64)
+    et = orc.w2v_exp_table().astype(np.float64)
+    M = (1 << 64) - 1
+    for blk, words in enumerate(sent):
+        nr = blk
+        for pos, word in enumerate(words):
+            nr = (nr * 2514903917 + 11) & M
+            b = nr % window
+            for a in range(b, 2 * window + 1 - b):
+                if a == window:
+                    continue
+                c = pos - window + a
+                if c < 0 or c >= len(words):
+                    continue
+                last = words[c]
+                neu1e = np.zeros(s0.shape[1])
+                for dd in range(negative + 1):
+                    if dd == 0:
+                        target, label = word, 1.0
+                    else:
+                        nr = (nr * 25214903917 + 11) & M
+                        tr = int(table[(nr >> 16) % len(table)])
+                        if tr == 0:
+                            tr = nr % (vocab_size - 1) + 1
+                        target = int(order[tr])
+                        if target == word:
+                            continue
+                        label = 0.0
+                    f = s0[last] @ s1[target]
+                    if f > 6:
+                        g = (label - 1) * alpha
+                    elif f < -6:
+                        g = label * alpha
+                    else:
+                        g = (label - et[int((f + 6) * 83)]) * alpha
+                    neu1e += g * s1[target]
+                    s1[target] = s1[target] + g * s0[last]
+                s0[last] = s0[last] + neu1e
+    return s0, s1
+
+
+def test_w2v_train_with_negatives_and_table_matches_fp64_restatement():
+    """orc_w2v_train with negatives and the sigmoid table (one thread, no
+    subsampling) against an FP64 restatement driven by the same LCG draws;
+    logits near a table-cell boundary may pick the neighbouring cell, so the
+    check is 1e-4 on every element."""
+    rng = np.random.default_rng(21)
+    V, d = 12, 8
+    deg = np.array([9, 8, 8, 7, 6, 6, 5, 5, 4, 3, 3, 2], np.uint32)
+    order, rank, inv, vs, tw = orc.vocab(deg, 1)
+    syn0 = (rng.standard_normal((V, d)) * 0.4).astype(np.float32)
+    syn1 = (rng.standard_normal((V, d)) * 0.4).astype(np.float32)
+    sent = [rng.integers(0, V, 9).tolist(), rng.integers(0, V, 14).tolist(), rng.integers(0, V, 6).tolist()]
+    offsets = np.cumsum([0] + [len(s) for s in sent]).astype(np.uint64)
+    tokens = np.array(sum(sent, []), np.uint32)
+    T = 1000
+    table = orc.w2v_unigram_table(deg[order], T)
+    s0, s1 = syn0.copy(), syn1.copy()
+    words, pairs = orc.w2v_train(s0, s1, offsets, tokens, deg, order, rank, inv, vs, tw, window=2, negative=3,
+                                 sample=0.0, alpha=0.05, use_lut=True, table_size=T, threads=1)
+    assert words == len(tokens) and pairs > 0
+    r0, r1 = _w2v_fp64(syn0, syn1, sent, order, 2, 3, 0.05, table, vs)
+    assert np.abs(s0 - r0).max() < 1e-4 and np.abs(s1 - r1).max() < 1e-4
+    assert np.abs(r1 - syn1).max() > 1e-3  # the negatives did train
