@@ -256,10 +256,12 @@ felog_status felog_build(felog_ctx *c) {
         TRY(c, hipMalloc(&c->d_edst, 4));
         TRY(c, hipMalloc(&c->d_ew, 4));
     }
-    TRY(c, build_csr(c));
-    std::vector<uint32_t> deg;
-    TRY(c, build_vocab_and_tables(c, deg));
-    TRY(c, build_alias(c));
+    uint32_t *edge_src = nullptr;
+    hipError_t be = build_csr(c, &edge_src);
+    if (be == hipSuccess) be = build_vocab_and_tables(c);
+    if (be == hipSuccess) be = build_alias(c, edge_src);
+    dfree(edge_src);
+    TRY(c, be);
     TRY(c, init_tables(c));
     const uint32_t V = c->V;
     TRY(c, hipMalloc(&c->d_flags, V));

@@ -193,9 +193,11 @@ enum : int {
 // graph_kernels.hip
 hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w,
                            uint64_t m, bool undirected);
-hipError_t build_csr(felog_ctx *c);
-hipError_t build_alias(felog_ctx *c);
-hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg_host);
+// CSR; *edge_src receives the vertex of every CSR entry (device, E
+// entries, owned by the caller) for the alias build.
+hipError_t build_csr(felog_ctx *c, uint32_t **edge_src);
+hipError_t build_alias(felog_ctx *c, const uint32_t *edge_src);
+hipError_t build_vocab_and_tables(felog_ctx *c);
 // walk_kernel.hip
 // Round state: eager calls set the round, its learning rate (alpha < 0: the
 // schedule's) and, unless ~0u, the sentence count; captured rounds advance
@@ -228,6 +230,14 @@ hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_t
 // ms[0] = whole training pass, ms[1] = prep kernel, ms[2] = tile kernel
 hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent,
                     uint64_t n_tok, const SgnsDebug *dbg, float *ms);
+// The same with the tile descriptors in a caller-provided device buffer of
+// tiles_cap descriptors (at least sgns_tiles_needed()); allocates nothing.
+uint64_t sgns_tiles_needed(const felog_ctx *c, uint64_t n_sent, uint64_t n_tok);
+constexpr uint64_t kSgnsDescBytes = 640;
+hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
+                             hipEvent_t mid, void *tiles, uint64_t tiles_cap);
+hipError_t run_sgns_into(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
+                         float *ms, void *tiles, uint64_t tiles_cap);
 // eval_kernel.hip
 // gated: evaluate only when the round state says so (captured rounds)
 hipError_t eval_enqueue(felog_ctx *c, bool gated);

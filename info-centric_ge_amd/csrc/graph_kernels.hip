@@ -9,9 +9,12 @@
 //   vocabulary order  src/word2vec.cu:697-755, 834-862 (degree descending)
 //   syn0 init (LCG)   src/word2vec.cu:864-902
 // Alias tables are new (the reference declares AliasBucket,
-// include/walk.hpp:87-103, but never builds one).
+// include/walk.hpp:87-103, but never builds one): an exact integer
+// construction, parallel over columns (below).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -42,12 +45,6 @@ __global__ void k_undirect(const uint32_t *__restrict__ src, const uint32_t *__r
             osrc[i] = s; odst[i] = d; ow[i] = x;
         }
     }
-}
-
-__global__ void k_degree(const uint32_t *__restrict__ src, uint64_t E, uint32_t *__restrict__ deg) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E; i += stride)
-        atomicAdd(&deg[src[i]], 1u);
 }
 
 __global__ void k_pack(const uint32_t *__restrict__ dst, const float *__restrict__ w, uint64_t E,
@@ -100,67 +97,6 @@ __global__ void k_edge_vr(const VertexRec *__restrict__ vwalk, uint64_t E, EdgeR
     }
 }
 
-// Threshold of an alias column: P(keep column) = thresh / 2^32.
-__device__ __forceinline__ uint32_t alias_quantise(double p) {
-    if (p >= 1.0) return 0xFFFFFFFFu;
-    if (!(p > 0.0)) return 0u;
-    return (uint32_t)(p * 4294967296.0);
-}
-
-// Vose's alias construction, one thread per vertex.  Vertices are taken in
-// degree-descending order so the lanes of a wavefront have similar work.
-// The two work stacks share the segment stk[b, b+n): small entries grow
-// upward from b, large entries downward from b+n-1.  Pops take the most
-// recent push.  FP64 throughout: weights are summed in column order, then
-// p_j = w_j * n / sum, and each pairing updates p_l = (p_l + p_s) - 1.
-__global__ void k_alias(const uint32_t *__restrict__ order, uint32_t V,
-                        const uint64_t *__restrict__ row_ptr, const float *__restrict__ w,
-                        const uint32_t *__restrict__ col, double *__restrict__ p,
-                        uint32_t *__restrict__ stk, EdgeRec *__restrict__ erec) {
-#pragma clang fp contract(off)
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        const uint32_t v = order[i];
-        const uint64_t b = row_ptr[v];
-        const uint32_t n = (uint32_t)(row_ptr[v + 1] - b);
-        if (n == 0) continue;
-        double sum = 0.0;
-        for (uint32_t j = 0; j < n; ++j) {
-            const float x = w[b + j];
-            sum = sum + (x > 0.f ? (double)x : 0.0);
-        }
-        uint32_t ns = 0, nl = 0;
-        const double dn = (double)n;
-        for (uint32_t j = 0; j < n; ++j) {
-            const float x = w[b + j];
-            const double pj = sum > 0.0 ? ((x > 0.f ? (double)x : 0.0) * dn) / sum : 1.0;
-            p[b + j] = pj;
-            if (pj < 1.0) stk[b + ns++] = j;
-            else stk[b + n - 1 - nl++] = j;
-        }
-        while (ns > 0 && nl > 0) {
-            const uint32_t s = stk[b + --ns];
-            const uint32_t l = stk[b + n - nl];
-            --nl;
-            const double ps = p[b + s];
-            erec[b + s] = EdgeRec{alias_quantise(ps), col[b + s], col[b + l], l, {}, {}};
-            const double pl = (p[b + l] + ps) - 1.0;
-            p[b + l] = pl;
-            if (pl < 1.0) stk[b + ns++] = l;
-            else stk[b + n - 1 - nl++] = l;
-        }
-        while (nl > 0) {
-            const uint32_t l = stk[b + n - nl];
-            --nl;
-            erec[b + l] = EdgeRec{0xFFFFFFFFu, col[b + l], col[b + l], l, {}, {}};
-        }
-        while (ns > 0) {
-            const uint32_t s = stk[b + --ns];
-            erec[b + s] = EdgeRec{0xFFFFFFFFu, col[b + s], col[b + s], s, {}, {}};
-        }
-    }
-}
-
 // syn0 initialisation of src/word2vec.cu:893-896: element e of the
 // vocabulary-ordered table takes x_{e+1} of the LCG x <- x*25214903917 + 11
 // started at 1, as ((x & 0xFFFF)/65536 - 0.5)/d.  jump[k] holds the affine
@@ -199,6 +135,403 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap) {
     return (unsigned)g;
 }
 
+// ---------------------------------------------------------------------------
+// Exact alias tables over many weight segments at once (a vertex's CSR row,
+// or the whole vocabulary for the negative sampler).  For a segment of n
+// columns with weights w_j (negatives count as 0):
+//   S    canonical FP64 sum: 4096-column chunks, each the xor-butterfly of 64
+//        lane partials (lane l sums columns l, l+64, ... in order); the
+//        chunk sums are combined by the same rule;
+//   q_j  trunc(((w_j n) / S) 2^32), an integer share of n 2^32; the
+//        remainder goes to the largest q (lowest column on ties); S = 0
+//        gives every column exactly 2^32;
+//   pairing  the integer sweep: light columns (q < 2^32) in order take the
+//        heavy column (q >= 2^32) current at their position as alias, and
+//        a heavy column whose residual falls below 2^32 becomes a bucket
+//        aliased to the next heavy column.  With the light deficits
+//        d = 2^32 - q and heavy surpluses s = q - 2^32 as prefix sums D, S
+//        (per segment, in column order) the sweep's state at any column is
+//        a function of D and S, so every column is paired independently:
+//          light with exclusive deficit prefix D: alias = the first heavy
+//            column whose inclusive surplus prefix is >= D (D = 0: the
+//            first heavy column), threshold q;
+//          heavy with inclusive surplus prefix S: if some light's
+//            inclusive deficit prefix exceeds S (and a later heavy exists),
+//            threshold 2^32 + S - (the first such prefix), alias = the next
+//            heavy column; else a whole bucket.
+// All integer steps are exact, so any evaluation order gives the tables of
+// the sequential sweep bit for bit.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kUnit = 1ull << 32;
+constexpr uint64_t kHeavyBit = 1ull << 63;
+constexpr uint32_t kChunk = 4096;
+
+__device__ __forceinline__ double wclamp(float x) { return x > 0.f ? (double)x : 0.0; }
+__device__ __forceinline__ double wclamp(double x) { return x > 0.0 ? x : 0.0; }
+
+// The xor butterfly over the 64 lanes (every lane ends with the same sum).
+__device__ __forceinline__ double butterfly(double p) {
+#pragma clang fp contract(off)
+    for (int o = 32; o > 0; o >>= 1) p = p + __shfl_xor(p, o);
+    return p;
+}
+
+// Lane partial of the canonical sum over x[0, n) (n <= 4096 for a chunk).
+template <typename W>
+__device__ __forceinline__ double lane_partial(const W *__restrict__ x, uint64_t n, uint32_t lane) {
+#pragma clang fp contract(off)
+    double p = 0.0;
+    for (uint64_t j = lane; j < n; j += 64) p = p + wclamp(x[j]);
+    return p;
+}
+
+__device__ __forceinline__ uint64_t share(double x, double dn, double S) {
+#pragma clang fp contract(off)
+    return (uint64_t)(((x * dn) / S) * 4294967296.0);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// (q, j) of the larger q, the lower j on ties.
+__device__ __forceinline__ void max_lowest(uint64_t &q, uint64_t &j, uint64_t q2, uint64_t j2) {
+    if (q2 > q || (q2 == q && j2 < j)) {
+        q = q2;
+        j = j2;
+    }
+}
+
+__device__ __forceinline__ void wave_argmax(uint64_t &q, uint64_t &j) {
+    for (int o = 32; o > 0; o >>= 1) max_lowest(q, j, __shfl_xor(q, o), __shfl_xor(j, o));
+}
+
+// Encoded column: light -> its deficit, heavy -> its surplus | kHeavyBit.
+__device__ __forceinline__ uint64_t encode(uint64_t q) { return q < kUnit ? kUnit - q : (q - kUnit) | kHeavyBit; }
+
+// Segments of at most one chunk: one wavefront each, sum, shares and the
+// encoded columns in one pass per step.
+template <typename W>
+__global__ __launch_bounds__(256) void k_alias_small(const uint64_t *__restrict__ seg, uint32_t nseg,
+                                                     const W *__restrict__ w, uint64_t *__restrict__ val) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t s = wave; s < nseg; s += nwaves) {
+        const uint64_t b = seg[s], n = seg[s + 1] - b;
+        if (n == 0 || n > kChunk) continue;
+        const double S = butterfly(lane_partial(w + b, n, lane));
+        const double dn = (double)n;
+        uint64_t sq = 0, qm = 0, jm = ~0ull;
+        if (S > 0.0) {
+            for (uint64_t j = lane; j < n; j += 64) {
+                const uint64_t q = share(wclamp(w[b + j]), dn, S);
+                sq += q;
+                max_lowest(qm, jm, q, j);
+            }
+            sq = wave_sum_u64(sq);
+            wave_argmax(qm, jm);
+        }
+        const uint64_t rem = n * kUnit - sq;
+        for (uint64_t j = lane; j < n; j += 64) {
+            uint64_t q = kUnit;
+            if (S > 0.0) q = share(wclamp(w[b + j]), dn, S) + (j == jm ? rem : 0ull);
+            val[b + j] = encode(q);
+        }
+    }
+}
+
+// Larger segments ("hubs", hub k = segment hub[k]) are cut into chunks;
+// chunk t of the launch belongs to the hub k with cstart[k] <= t <
+// cstart[k+1].
+__device__ __forceinline__ uint32_t hub_of_chunk(const uint64_t *__restrict__ cstart, uint32_t nh, uint64_t t) {
+    uint32_t lo = 0, hi = nh;  // last k with cstart[k] <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (cstart[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_alias_chunk_sum(const uint64_t *__restrict__ seg,
+                                                         const uint32_t *__restrict__ hub,
+                                                         const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                         const W *__restrict__ w, double *__restrict__ csum) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nchunks = cstart[nh];
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < nchunks; t += nwaves) {
+        const uint32_t k = hub_of_chunk(cstart, nh, t);
+        const uint64_t b = seg[hub[k]], n = seg[hub[k] + 1] - b;
+        const uint64_t c0 = (t - cstart[k]) * kChunk;
+        const uint64_t cn = n - c0 < kChunk ? n - c0 : kChunk;
+        const double p = butterfly(lane_partial(w + b + c0, cn, lane));
+        if (lane == 0) csum[t] = p;
+    }
+}
+
+// One wavefront per hub: the canonical sum of its chunk sums.
+__global__ __launch_bounds__(256) void k_alias_hub_sum(const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                       const double *__restrict__ csum, double *__restrict__ hsum) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nh; k += nwaves) {
+        const double p = butterfly(lane_partial(csum + cstart[k], cstart[k + 1] - cstart[k], lane));
+        if (lane == 0) hsum[k] = p;
+    }
+}
+
+// Per chunk: the sum of its shares and its largest share (lowest column).
+template <typename W>
+__global__ __launch_bounds__(256) void k_alias_chunk_q(const uint64_t *__restrict__ seg,
+                                                       const uint32_t *__restrict__ hub,
+                                                       const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                       const W *__restrict__ w, const double *__restrict__ hsum,
+                                                       uint64_t *__restrict__ cq) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nchunks = cstart[nh];
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < nchunks; t += nwaves) {
+        const uint32_t k = hub_of_chunk(cstart, nh, t);
+        const uint64_t b = seg[hub[k]], n = seg[hub[k] + 1] - b;
+        const uint64_t c0 = (t - cstart[k]) * kChunk;
+        const uint64_t cn = n - c0 < kChunk ? n - c0 : kChunk;
+        const double S = hsum[k], dn = (double)n;
+        uint64_t sq = 0, qm = 0, jm = ~0ull;
+        if (S > 0.0) {
+            for (uint64_t j = c0 + lane; j < c0 + cn; j += 64) {
+                const uint64_t q = share(wclamp(w[b + j]), dn, S);
+                sq += q;
+                max_lowest(qm, jm, q, j);
+            }
+            sq = wave_sum_u64(sq);
+            wave_argmax(qm, jm);
+        }
+        if (lane == 0) {
+            cq[3 * t] = sq;
+            cq[3 * t + 1] = qm;
+            cq[3 * t + 2] = jm;
+        }
+    }
+}
+
+// One wavefront per hub: remainder and the column that takes it.
+__global__ __launch_bounds__(256) void k_alias_hub_q(const uint64_t *__restrict__ seg, const uint32_t *__restrict__ hub,
+                                                     const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                     const uint64_t *__restrict__ cq, uint64_t *__restrict__ hrem) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nh; k += nwaves) {
+        uint64_t sq = 0, qm = 0, jm = ~0ull;
+        for (uint64_t t = cstart[k] + lane; t < cstart[k + 1]; t += 64) {
+            sq += cq[3 * t];
+            max_lowest(qm, jm, cq[3 * t + 1], cq[3 * t + 2]);
+        }
+        sq = wave_sum_u64(sq);
+        wave_argmax(qm, jm);
+        if (lane == 0) {
+            const uint64_t n = seg[hub[k] + 1] - seg[hub[k]];
+            hrem[2 * k] = n * kUnit - sq;
+            hrem[2 * k + 1] = jm;
+        }
+    }
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_alias_chunk_val(const uint64_t *__restrict__ seg,
+                                                         const uint32_t *__restrict__ hub,
+                                                         const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                         const W *__restrict__ w, const double *__restrict__ hsum,
+                                                         const uint64_t *__restrict__ hrem,
+                                                         uint64_t *__restrict__ val) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nchunks = cstart[nh];
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < nchunks; t += nwaves) {
+        const uint32_t k = hub_of_chunk(cstart, nh, t);
+        const uint64_t b = seg[hub[k]], n = seg[hub[k] + 1] - b;
+        const uint64_t c0 = (t - cstart[k]) * kChunk;
+        const uint64_t cn = n - c0 < kChunk ? n - c0 : kChunk;
+        const double S = hsum[k], dn = (double)n;
+        const uint64_t rem = hrem[2 * k], jm = hrem[2 * k + 1];
+        for (uint64_t j = c0 + lane; j < c0 + cn; j += 64) {
+            uint64_t q = kUnit;
+            if (S > 0.0) q = share(wclamp(w[b + j]), dn, S) + (j == jm ? rem : 0ull);
+            val[b + j] = encode(q);
+        }
+    }
+}
+
+__global__ void k_alias_hub_chunks(const uint64_t *__restrict__ seg, const uint32_t *__restrict__ hub, uint32_t nh,
+                                   uint64_t *__restrict__ cnt) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= nh; k += stride)
+        cnt[k] = k < nh ? (seg[hub[k] + 1] - seg[hub[k]] + kChunk - 1) / kChunk : 0;
+}
+
+struct LightDeficit {
+    __host__ __device__ uint64_t operator()(uint64_t v) const { return (v & kHeavyBit) ? 0ull : v; }
+};
+struct HeavySurplus {
+    __host__ __device__ uint64_t operator()(uint64_t v) const { return (v & kHeavyBit) ? (v & ~kHeavyBit) : 0ull; }
+};
+struct LightCount {
+    __host__ __device__ uint32_t operator()(uint64_t v) const { return (v & kHeavyBit) ? 0u : 1u; }
+};
+struct IsHub {
+    const uint64_t *seg;
+    __host__ __device__ bool operator()(uint32_t s) const { return seg[s + 1] - seg[s] > kChunk; }
+};
+
+// First index i in [lo, hi) with pred(i) true (pred false then true).
+template <typename P>
+__device__ __forceinline__ uint64_t first_true(uint64_t lo, uint64_t hi, P pred) {
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (pred(mid)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Pairing of every column (flat over the columns of all segments).
+// seg_of[e] is the segment of column e (null: one segment).  out(e, j,
+// threshold, alias column) stores the result.
+template <typename Out>
+__global__ __launch_bounds__(256) void k_alias_pair(const uint64_t *__restrict__ seg, const uint32_t *__restrict__ seg_of,
+                                                    uint64_t total, const uint64_t *__restrict__ val,
+                                                    const uint64_t *__restrict__ LD, const uint64_t *__restrict__ SH,
+                                                    const uint32_t *__restrict__ NL, Out out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const uint32_t s = seg_of ? seg_of[e] : 0u;
+        const uint64_t b = seg[s], end = seg[s + 1];
+        const uint64_t bL = b ? LD[b - 1] : 0ull, bS = b ? SH[b - 1] : 0ull;
+        const uint32_t bN = b ? NL[b - 1] : 0u;
+        // heavy columns in [b, i]
+        auto heavies = [&](uint64_t i) -> uint64_t { return (i - b + 1) - (uint64_t)(NL[i] - bN); };
+        const uint64_t v = val[e];
+        uint32_t thr = 0xFFFFFFFFu;
+        uint64_t ali = e;
+        if (!(v & kHeavyBit)) {
+            const uint64_t D = (LD[e] - bL) - v;  // deficits of the lights before e
+            ali = D == 0 ? first_true(b, end, [&](uint64_t i) { return heavies(i) >= 1; })
+                         : first_true(b, end, [&](uint64_t i) { return SH[i] - bS >= D; });
+            thr = (uint32_t)(kUnit - v);
+        } else {
+            const uint64_t S = SH[e] - bS;
+            const uint64_t h = heavies(e);  // 1-based rank of e among the heavy columns
+            const uint64_t nH = heavies(end - 1);
+            if (h < nH && LD[end - 1] - bL > S) {
+                const uint64_t i = first_true(b, end, [&](uint64_t x) { return LD[x] - bL > S; });
+                thr = (uint32_t)(kUnit + S - (LD[i] - bL));
+                ali = first_true(e + 1, end, [&](uint64_t x) { return heavies(x) >= h + 1; });
+            }
+        }
+        out(e, e - b, thr, ali - b);
+    }
+}
+
+// Outputs: the walk's edge records, or the negative sampler's {thr, alias}.
+struct EdgeOut {
+    const uint64_t *seg;
+    const uint32_t *col;
+    EdgeRec *erec;
+    __device__ void operator()(uint64_t e, uint64_t j, uint32_t thr, uint64_t a) const {
+        const uint64_t b = e - j;
+        erec[e] = EdgeRec{thr, col[e], col[b + a], (uint32_t)a, {}, {}};
+    }
+};
+struct PairOut {
+    uint2 *out;
+    __device__ void operator()(uint64_t e, uint64_t, uint32_t thr, uint64_t a) const {
+        out[e] = make_uint2(thr, (uint32_t)a);
+    }
+};
+
+// Sort keys of the vocabulary (complemented degree, ids), the totals
+// cleared and the one-segment offsets {0, V} of the negative sampler.
+__global__ void k_vocab_keys(const uint32_t *__restrict__ deg, uint32_t V, uint32_t *__restrict__ keys,
+                             uint32_t *__restrict__ ids, uint64_t *__restrict__ totals, uint64_t *__restrict__ seg01) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        totals[0] = totals[1] = 0;
+        seg01[0] = 0;
+        seg01[1] = V;
+    }
+    for (uint32_t v = t; v < V; v += stride) {
+        keys[v] = ~deg[v];
+        ids[v] = v;
+    }
+}
+
+// rank, vocabulary flag (degree >= min-count, or rank 0), the negative
+// sampler's weight deg^0.75 = sqrt(deg) * sqrt(sqrt(deg)) (correctly
+// rounded square roots: the host computes the same bits), and the totals
+// (vocabulary size, train words = sum of the vocabulary's degrees).
+__global__ void k_vocab_rank(const uint32_t *__restrict__ order, const uint32_t *__restrict__ deg, uint32_t V,
+                             uint32_t min_count, uint32_t *__restrict__ rank, uint8_t *__restrict__ inv,
+                             double *__restrict__ uw, unsigned long long *totals) {
+#pragma clang fp contract(off)
+    const uint32_t stride = gridDim.x * blockDim.x;
+    unsigned long long vs = 0, tw = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
+        const uint32_t v = order[i];
+        const uint32_t d = deg[v];
+        const bool in = d >= min_count || i == 0;
+        rank[v] = i;
+        inv[v] = in ? 1 : 0;
+        const double x = (double)d;
+        uw[v] = in ? __builtin_sqrt(x) * __builtin_sqrt(__builtin_sqrt(x)) : 0.0;
+        vs += in ? 1ull : 0ull;
+        tw += in ? d : 0ull;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        vs += __shfl_xor(vs, o);
+        tw += __shfl_xor(tw, o);
+    }
+    if ((threadIdx.x & 63) == 0 && (vs | tw)) {
+        atomicAdd(&totals[0], vs);
+        atomicAdd(&totals[1], tw);
+    }
+}
+
+// Subsampling keep thresholds: keep with probability
+// min(1, (sqrt(c/sT) + 1) sT/c), c = degree, sT = sample * train words, on
+// the u32 scale; 0 outside the vocabulary (one gather per token decides).
+__global__ void k_keep(const uint32_t *__restrict__ deg, const uint8_t *__restrict__ inv, uint32_t V, float sample,
+                       const uint64_t *__restrict__ train_words, uint32_t *__restrict__ keep) {
+#pragma clang fp contract(off)
+    const uint64_t tw = *train_words;
+    const double sT = (double)sample * (double)tw;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+        uint32_t k = 0xFFFFFFFFu;
+        const double cnt = (double)deg[v];
+        if (sample > 0.f && tw > 0 && cnt > 0.0) {
+            const double prob = (__builtin_sqrt(cnt / sT) + 1.0) * (sT / cnt);
+            if (prob < 1.0) k = prob <= 0.0 ? 0u : (uint32_t)(prob * 4294967296.0);
+        }
+        keep[v] = inv[v] ? k : 0u;
+    }
+}
+
+// row_ptr from the sorted sources: entry e starts every vertex v with
+// key[e-1] < v <= key[e] (key[-1] = -1, key[E] = V), so vertices without
+// edges get empty rows.
+__global__ void k_row_ptr(const uint32_t *__restrict__ key, uint64_t E, uint32_t V, uint64_t *__restrict__ row_ptr) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e <= E; e += stride) {
+        const uint64_t lo = e ? (uint64_t)key[e - 1] + 1 : 0;
+        const uint64_t hi = e < E ? (uint64_t)key[e] : (uint64_t)V;
+        for (uint64_t v = lo; v <= hi; ++v) row_ptr[v] = e;
+    }
+}
+
 }  // namespace
 
 #define HIP_TRY(x)                              \
@@ -234,13 +567,95 @@ hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *ds
     return hipSuccess;
 }
 
-hipError_t build_csr(felog_ctx *c) {
+// Exact alias tables of nseg weight segments: seg holds nseg + 1 column
+// offsets on the device (total = the last one), w the weights, seg_of the
+// segment of every column (null when nseg == 1); out stores each column's
+// threshold and alias.  Temporary device memory: 28 bytes per column.
+template <typename W, typename Out>
+static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nseg, uint64_t total, const W *w,
+                                 const uint32_t *seg_of, Out out) {
+    if (total == 0 || nseg == 0) return hipSuccess;
+    hipStream_t st = c->stream;
+    uint64_t *val = nullptr, *LD = nullptr, *SH = nullptr;
+    uint32_t *NL = nullptr, *hub = nullptr, *d_nh = nullptr;
+    HIP_TRY(hipMalloc(&val, total * 8));
+    HIP_TRY(hipMalloc(&LD, total * 8));
+    HIP_TRY(hipMalloc(&SH, total * 8));
+    HIP_TRY(hipMalloc(&NL, total * 4));
+    HIP_TRY(hipMalloc(&hub, (uint64_t)nseg * 4));
+    HIP_TRY(hipMalloc(&d_nh, 8));
+    k_alias_small<W><<<grid_for((uint64_t)nseg * 64, 256, 1u << 16), 256, 0, st>>>(seg, nseg, w, val);
+    HIP_TRY(hipGetLastError());
+    // segments longer than one chunk
+    size_t tb = 0;
+    const rocprim::counting_iterator<uint32_t> ids(0);
+    HIP_TRY(hipcub::DeviceSelect::If(nullptr, tb, ids, hub, d_nh, nseg, IsHub{seg}, st));
+    size_t tb2 = 0, tb3 = 0, tb4 = 0;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb2, rocprim::make_transform_iterator(val, LightDeficit{}), LD,
+                                             total, st));
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tb3, rocprim::make_transform_iterator(val, LightCount{}), NL,
+                                             total, st));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                             (uint64_t)nseg + 1, st));
+    size_t tmax = tb;
+    for (size_t x : {tb2, tb3, tb4}) tmax = x > tmax ? x : tmax;
+    void *tmp = nullptr;
+    HIP_TRY(hipMalloc(&tmp, tmax + 16));
+    tb = tmax;
+    HIP_TRY(hipcub::DeviceSelect::If(tmp, tb, ids, hub, d_nh, nseg, IsHub{seg}, st));
+    uint32_t nh = 0;
+    HIP_TRY(hipMemcpyAsync(&nh, d_nh, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    uint64_t *cnt = nullptr, *cstart = nullptr, *cq = nullptr, *hrem = nullptr;
+    double *csum = nullptr, *hsum = nullptr;
+    if (nh) {
+        HIP_TRY(hipMalloc(&cnt, ((uint64_t)nh + 1) * 8));
+        HIP_TRY(hipMalloc(&cstart, ((uint64_t)nh + 1) * 8));
+        k_alias_hub_chunks<<<grid_for((uint64_t)nh + 1, 256, 4096), 256, 0, st>>>(seg, hub, nh, cnt);
+        HIP_TRY(hipGetLastError());
+        tb = tmax;
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, cstart, (uint64_t)nh + 1, st));
+        uint64_t nchunks = 0;
+        HIP_TRY(hipMemcpyAsync(&nchunks, cstart + nh, 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipMalloc(&csum, nchunks * 8));
+        HIP_TRY(hipMalloc(&cq, nchunks * 24));
+        HIP_TRY(hipMalloc(&hsum, (uint64_t)nh * 8));
+        HIP_TRY(hipMalloc(&hrem, (uint64_t)nh * 16));
+        const unsigned gc = grid_for(nchunks * 64, 256, 1u << 16), gh = grid_for((uint64_t)nh * 64, 256, 1u << 16);
+        k_alias_chunk_sum<W><<<gc, 256, 0, st>>>(seg, hub, cstart, nh, w, csum);
+        k_alias_hub_sum<<<gh, 256, 0, st>>>(cstart, nh, csum, hsum);
+        k_alias_chunk_q<W><<<gc, 256, 0, st>>>(seg, hub, cstart, nh, w, hsum, cq);
+        k_alias_hub_q<<<gh, 256, 0, st>>>(seg, hub, cstart, nh, cq, hrem);
+        k_alias_chunk_val<W><<<gc, 256, 0, st>>>(seg, hub, cstart, nh, w, hsum, hrem, val);
+        HIP_TRY(hipGetLastError());
+    }
+    // per-segment prefixes come from global ones minus the value before the
+    // segment's first column: every value is an exact integer
+    tb = tmax;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tb, rocprim::make_transform_iterator(val, LightDeficit{}), LD,
+                                             total, st));
+    tb = tmax;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tb, rocprim::make_transform_iterator(val, HeavySurplus{}), SH,
+                                             total, st));
+    tb = tmax;
+    HIP_TRY(hipcub::DeviceScan::InclusiveSum(tmp, tb, rocprim::make_transform_iterator(val, LightCount{}), NL,
+                                             total, st));
+    k_alias_pair<Out><<<grid_for(total, 256, 1u << 20), 256, 0, st>>>(seg, seg_of, total, val, LD, SH, NL, out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    for (void *p : {(void *)val, (void *)LD, (void *)SH, (void *)NL, (void *)hub, (void *)d_nh, tmp, (void *)cnt,
+                    (void *)cstart, (void *)cq, (void *)hrem, (void *)csum, (void *)hsum})
+        dfree(p);
+    return hipSuccess;
+}
+
+hipError_t build_csr(felog_ctx *c, uint32_t **edge_src) {
     const uint64_t E = c->E;
     const uint32_t V = c->V;
     HIP_TRY(hipMalloc(&c->d_row_ptr, (uint64_t)(V + 1) * 8));
     HIP_TRY(hipMalloc(&c->d_deg, (uint64_t)V * 4));
     HIP_TRY(hipMalloc(&c->d_vrec, (uint64_t)V * sizeof(VertexRec)));
-    HIP_TRY(hipMemsetAsync(c->d_deg, 0, (uint64_t)V * 4, c->stream));
     HIP_TRY(hipMalloc(&c->d_col, E * 4 + 4));
     HIP_TRY(hipMalloc(&c->d_w, E * 4 + 4));
 
@@ -262,28 +677,21 @@ hipError_t build_csr(felog_ctx *c) {
     void *d_tmp = nullptr;
     HIP_TRY(hipMalloc(&d_tmp, tmp + 16));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp, keys, vals, E, 0, bits, c->stream));
-    k_degree<<<grid_for(E, 256, 65535), 256, 0, c->stream>>>(keys.Current(), E, c->d_deg);
+    // row_ptr from the sorted sources; degrees and vertex records from it
+    k_row_ptr<<<grid_for(E + 1, 256, 65535), 256, 0, c->stream>>>(keys.Current(), E, V, c->d_row_ptr);
     HIP_TRY(hipGetLastError());
     k_unpack<<<grid_for(E, 256, 65535), 256, 0, c->stream>>>(vals.Current(), E, c->d_col, c->d_w);
     HIP_TRY(hipGetLastError());
-    // row_ptr = exclusive prefix sum of the degrees (u64).
-    HIP_TRY(hipMemsetAsync(c->d_row_ptr, 0, 8, c->stream));
-    size_t tmp2 = 0;
-    HIP_TRY(hipcub::DeviceScan::InclusiveSum(nullptr, tmp2, c->d_deg, c->d_row_ptr + 1, (uint64_t)V,
-                                             c->stream));
-    void *d_tmp2 = nullptr;
-    HIP_TRY(hipMalloc(&d_tmp2, tmp2 + 16));
-    HIP_TRY(hipcub::DeviceScan::InclusiveSum(d_tmp2, tmp2, c->d_deg, c->d_row_ptr + 1, (uint64_t)V,
-                                             c->stream));
     k_vrec<<<grid_for(V, 256, 65535), 256, 0, c->stream>>>(c->d_row_ptr, V, c->d_vrec, c->d_deg);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
+    // the sorted sources are the vertex of every CSR entry: kept for the
+    // alias build, the other key buffer is released
+    *edge_src = keys.Current();
+    dfree(keys.Current() == c->d_esrc ? keys_b : c->d_esrc);
     dfree(d_tmp);
-    dfree(d_tmp2);
     dfree(vals_a);
     dfree(vals_b);
-    dfree(keys_b);
-    dfree(c->d_esrc);
     dfree(c->d_edst);
     dfree(c->d_ew);
     c->d_esrc = c->d_edst = nullptr;
@@ -291,52 +699,11 @@ hipError_t build_csr(felog_ctx *c) {
     return hipSuccess;
 }
 
-// Vocabulary: vertices by degree descending, ties by id ascending (stable
-// sort of ids 0..V-1 on the complemented degree).  The reference's qsort
-// leaves ties unspecified (src/word2vec.cu:737).
-static hipError_t sort_vocab(felog_ctx *c) {
-    const uint32_t V = c->V;
-    uint32_t *keys_a = nullptr, *keys_b = nullptr, *ids_a = nullptr, *ids_b = nullptr;
-    HIP_TRY(hipMalloc(&keys_a, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&keys_b, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&ids_a, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&ids_b, (uint64_t)V * 4));
-    std::vector<uint32_t> iota(V);
-    for (uint32_t i = 0; i < V; ++i) iota[i] = i;
-    HIP_TRY(hipMemcpyAsync(ids_a, iota.data(), (uint64_t)V * 4, hipMemcpyHostToDevice, c->stream));
-    // keys = ~deg
-    std::vector<uint32_t> deg(V);
-    HIP_TRY(hipMemcpyAsync(deg.data(), c->d_deg, (uint64_t)V * 4, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    for (uint32_t i = 0; i < V; ++i) deg[i] = ~deg[i];
-    HIP_TRY(hipMemcpyAsync(keys_a, deg.data(), (uint64_t)V * 4, hipMemcpyHostToDevice, c->stream));
-    hipcub::DoubleBuffer<uint32_t> keys(keys_a, keys_b), ids(ids_a, ids_b);
-    size_t tmp = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, ids, (uint64_t)V, 0, 32, c->stream));
-    void *d_tmp = nullptr;
-    HIP_TRY(hipMalloc(&d_tmp, tmp + 16));
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp, keys, ids, (uint64_t)V, 0, 32, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->d_vocab_order, ids.Current(), (uint64_t)V * 4, hipMemcpyDeviceToDevice,
-                           c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    dfree(d_tmp);
-    dfree(keys_a);
-    dfree(keys_b);
-    dfree(ids_a);
-    dfree(ids_b);
-    return hipSuccess;
-}
-
-hipError_t build_alias(felog_ctx *c) {
+hipError_t build_alias(felog_ctx *c, const uint32_t *edge_src) {
     const uint64_t E = c->E;
     HIP_TRY(hipMalloc(&c->d_erec, E * sizeof(EdgeRec) + 16));
-    double *p = nullptr;
-    uint32_t *stk = nullptr;
-    HIP_TRY(hipMalloc(&p, E * 8 + 8));
-    HIP_TRY(hipMalloc(&stk, E * 4 + 4));
-    k_alias<<<grid_for(c->V, 64, 1u << 20), 64, 0, c->stream>>>(c->d_vocab_order, c->V, c->d_row_ptr,
-                                                                c->d_w, c->d_col, p, stk, c->d_erec);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(alias_segments(c, c->d_row_ptr, c->V, E, (const float *)c->d_w, edge_src,
+                           EdgeOut{c->d_row_ptr, c->d_col, c->d_erec}));
     HIP_TRY(hipMalloc(&c->d_vwalk, (uint64_t)c->V * sizeof(VertexRec) + 8));
     k_vwalk<<<grid_for(c->V, 256, 65535), 256, 0, c->stream>>>(c->d_vrec, c->d_col, c->V, c->d_vwalk);
     HIP_TRY(hipGetLastError());
@@ -345,110 +712,57 @@ hipError_t build_alias(felog_ctx *c) {
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
-    dfree(p);
-    dfree(stk);
     return hipSuccess;
 }
 
-// Host Vose over V buckets (same pairing rule as k_alias), for the unigram
-// negative sampler: p_v proportional to deg^0.75 over vocabulary vertices
-// (src/word2vec.cu:674-694 draws from the same law through a 1e8 table).
-static void host_alias(const std::vector<double> &wts, std::vector<uint32_t> &thr,
-                       std::vector<uint32_t> &ali) {
-    const uint32_t n = (uint32_t)wts.size();
-    thr.assign(n, 0xFFFFFFFFu);
-    ali.resize(n);
-    double sum = 0.0;
-    for (uint32_t j = 0; j < n; ++j) sum = sum + (wts[j] > 0.0 ? wts[j] : 0.0);
-    std::vector<double> p(n);
-    std::vector<uint32_t> stk(n);
-    uint32_t ns = 0, nl = 0;
-    for (uint32_t j = 0; j < n; ++j) {
-        const double pj = sum > 0.0 ? ((wts[j] > 0.0 ? wts[j] : 0.0) * (double)n) / sum : 1.0;
-        p[j] = pj;
-        if (pj < 1.0) stk[ns++] = j;
-        else stk[n - 1 - nl++] = j;
-    }
-    auto q = [](double x) -> uint32_t {
-        if (x >= 1.0) return 0xFFFFFFFFu;
-        if (!(x > 0.0)) return 0u;
-        return (uint32_t)(x * 4294967296.0);
-    };
-    while (ns > 0 && nl > 0) {
-        const uint32_t s = stk[--ns];
-        const uint32_t l = stk[n - nl];
-        --nl;
-        thr[s] = q(p[s]);
-        ali[s] = l;
-        const double pl = (p[l] + p[s]) - 1.0;
-        p[l] = pl;
-        if (pl < 1.0) stk[ns++] = l;
-        else stk[n - 1 - nl++] = l;
-    }
-    while (nl > 0) {
-        const uint32_t l = stk[n - nl];
-        --nl;
-        thr[l] = 0xFFFFFFFFu;
-        ali[l] = l;
-    }
-    while (ns > 0) {
-        const uint32_t s = stk[--ns];
-        thr[s] = 0xFFFFFFFFu;
-        ali[s] = s;
-    }
-}
-
-hipError_t build_vocab_and_tables(felog_ctx *c, std::vector<uint32_t> &deg) {
+// Vocabulary (vertices by degree descending, ties by id ascending: a stable
+// sort of ids 0..V-1 on the complemented degree; the reference's qsort
+// leaves ties unspecified, src/word2vec.cu:737), the vocabulary flags and
+// totals (src/word2vec.cu:740-747), the negative sampler over deg^0.75
+// (src/word2vec.cu:674-694 draws from the same law through a 1e8 table) and
+// the subsampling thresholds (src/word2vec.cu:180-212), all on the device.
+hipError_t build_vocab_and_tables(felog_ctx *c) {
     const uint32_t V = c->V;
+    hipStream_t st = c->stream;
     HIP_TRY(hipMalloc(&c->d_vocab_order, (uint64_t)V * 4));
     HIP_TRY(hipMalloc(&c->d_vocab_rank, (uint64_t)V * 4));
     HIP_TRY(hipMalloc(&c->d_in_vocab, (uint64_t)V));
     HIP_TRY(hipMalloc(&c->d_uni, (uint64_t)V * 8));
     HIP_TRY(hipMalloc(&c->d_keep_thr, (uint64_t)V * 4));
-    HIP_TRY(sort_vocab(c));
-    std::vector<uint32_t> order(V), rank(V);
-    deg.resize(V);
-    HIP_TRY(hipMemcpy(order.data(), c->d_vocab_order, (uint64_t)V * 4, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(deg.data(), c->d_deg, (uint64_t)V * 4, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < V; ++i) rank[order[i]] = i;
-    // src/word2vec.cu:740-747: vocab keeps degree >= min-count plus index 0;
-    // train_words sums the kept degrees.
-    std::vector<uint8_t> inv(V);
-    uint64_t vs = 0, tw = 0;
-    for (uint32_t v = 0; v < V; ++v) {
-        inv[v] = (deg[v] >= c->opt.min_count || rank[v] == 0) ? 1 : 0;
-        if (inv[v]) { ++vs; tw += deg[v]; }
-    }
-    c->vocab_size = vs;
-    c->train_words = tw;
-    // unigram^0.75 over the vocabulary
-    std::vector<double> uw(V);
-    for (uint32_t v = 0; v < V; ++v) uw[v] = inv[v] ? std::pow((double)deg[v], 0.75) : 0.0;
-    std::vector<uint32_t> uthr, uali;
-    host_alias(uw, uthr, uali);
-    // subsampling keep thresholds (src/word2vec.cu:180-212): keep with
-    // probability min(1, (sqrt(c/sT)+1)*sT/c), c = degree, sT = sample*train_words.
-    std::vector<uint32_t> keep(V, 0xFFFFFFFFu);
-    const double sT = (double)c->opt.sample * (double)tw;
-    if (c->opt.sample > 0.f && tw > 0) {
-        for (uint32_t v = 0; v < V; ++v) {
-            const double cnt = (double)deg[v];
-            if (cnt <= 0.0) continue;
-            double prob = (std::sqrt(cnt / sT) + 1.0) * (sT / cnt);
-            if (prob < 1.0) keep[v] = prob <= 0.0 ? 0u : (uint32_t)(prob * 4294967296.0);
-        }
-    }
-    // one gather per token in the SGNS filter: vertices outside the
-    // vocabulary are never kept (threshold 0); one per negative: the
-    // sampler's threshold and alias side by side
-    for (uint32_t v = 0; v < V; ++v)
-        if (!inv[v]) keep[v] = 0;
-    std::vector<uint2> uni(V);
-    for (uint32_t v = 0; v < V; ++v) uni[v] = make_uint2(uthr[v], uali[v]);
-    HIP_TRY(hipMemcpy(c->d_vocab_rank, rank.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_in_vocab, inv.data(), (uint64_t)V, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_uni, uni.data(), (uint64_t)V * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_keep_thr, keep.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
+    uint32_t *keys_a = nullptr, *keys_b = nullptr, *ids_a = nullptr, *ids_b = nullptr;
+    uint64_t *totals = nullptr, *seg01 = nullptr;
+    double *uw = nullptr;
+    HIP_TRY(hipMalloc(&keys_a, (uint64_t)V * 4));
+    HIP_TRY(hipMalloc(&keys_b, (uint64_t)V * 4));
+    HIP_TRY(hipMalloc(&ids_a, (uint64_t)V * 4));
+    HIP_TRY(hipMalloc(&ids_b, (uint64_t)V * 4));
+    HIP_TRY(hipMalloc(&totals, 16));
+    HIP_TRY(hipMalloc(&seg01, 16));
+    HIP_TRY(hipMalloc(&uw, (uint64_t)V * 8));
+    k_vocab_keys<<<grid_for(V, 256, 65535), 256, 0, st>>>(c->d_deg, V, keys_a, ids_a, totals, seg01);
+    HIP_TRY(hipGetLastError());
+    hipcub::DoubleBuffer<uint32_t> keys(keys_a, keys_b), ids(ids_a, ids_b);
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, ids, (uint64_t)V, 0, 32, st));
+    void *d_tmp = nullptr;
+    HIP_TRY(hipMalloc(&d_tmp, tmp + 16));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp, keys, ids, (uint64_t)V, 0, 32, st));
+    HIP_TRY(hipMemcpyAsync(c->d_vocab_order, ids.Current(), (uint64_t)V * 4, hipMemcpyDeviceToDevice, st));
+    k_vocab_rank<<<grid_for(V, 256, 65535), 256, 0, st>>>(c->d_vocab_order, c->d_deg, V, c->opt.min_count,
+                                                         c->d_vocab_rank, c->d_in_vocab, uw,
+                                                         (unsigned long long *)totals);
+    HIP_TRY(hipGetLastError());
+    k_keep<<<grid_for(V, 256, 65535), 256, 0, st>>>(c->d_deg, c->d_in_vocab, V, c->opt.sample, totals + 1,
+                                                   c->d_keep_thr);
+    HIP_TRY(hipGetLastError());
+    uint64_t h_tot[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(h_tot, totals, 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(alias_segments(c, seg01, 1, V, (const double *)uw, nullptr, PairOut{c->d_uni}));
+    c->vocab_size = h_tot[0];
+    c->train_words = h_tot[1];
+    for (void *p : {(void *)keys_a, (void *)keys_b, (void *)ids_a, (void *)ids_b, (void *)totals, (void *)seg01,
+                    (void *)uw, d_tmp})
+        dfree(p);
     // n log2 n table for the walk statistics
     const uint32_t L = c->opt.max_walk_len;
     std::vector<double> T(L + 2), dT(L + 1);

@@ -271,9 +271,7 @@ uint32_t sgns_tile_rows(uint32_t centres, uint32_t window) {
 
 hipError_t sgns_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok) {
     using namespace sgns;
-    // every tile holds at least one centre and every sentence ends one tile
-    const uint32_t B = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
-    const uint64_t need = n_tok / B + n_sent + 64;
+    const uint64_t need = sgns_tiles_needed(c, n_sent, n_tok);
     if (need <= c->tiles_cap) return hipSuccess;
     drop_round_graph(c);
     dfree(c->d_tiles);
@@ -287,6 +285,11 @@ hipError_t sgns_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok) {
 
 hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
                         hipEvent_t mid) {
+    return sgns_enqueue_into(c, d_off, d_tok, dbg, mid, c->d_tiles, c->tiles_cap);
+}
+
+hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
+                             hipEvent_t mid, void *tiles, uint64_t tiles_cap) {
     using namespace sgns;
     Params a;
     a.off = d_off;
@@ -316,8 +319,8 @@ hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_t
     a.dbg_logits = dbg ? dbg->logits : nullptr;
     a.dbg_dv = dbg ? dbg->dv : nullptr;
     a.dbg_du = dbg ? dbg->du : nullptr;
-    a.tiles = (unsigned char *)c->d_tiles;
-    a.tiles_cap = c->tiles_cap;
+    a.tiles = (unsigned char *)tiles;
+    a.tiles_cap = tiles_cap;
     // the sentence count is read on the device: a fixed grid strides over it
     k_sgns_prep<<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
     HIP_TRY(hipGetLastError());
@@ -325,12 +328,25 @@ hipError_t sgns_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_t
     return launch_tile(c, a);
 }
 
+static_assert(kSgnsDescBytes == (uint64_t)sgns::kDescBytes, "tile descriptor size");
+
+uint64_t sgns_tiles_needed(const felog_ctx *c, uint64_t n_sent, uint64_t n_tok) {
+    // every tile holds at least one centre and every sentence ends one tile
+    const uint32_t B = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
+    return n_tok / B + n_sent + 64;
+}
+
 hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
                     const SgnsDebug *dbg, float *ms) {
     HIP_TRY(sgns_reserve(c, n_sent, n_tok));
+    return run_sgns_into(c, d_off, d_tok, dbg, ms, c->d_tiles, c->tiles_cap);
+}
+
+hipError_t run_sgns_into(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
+                         float *ms, void *tiles, uint64_t tiles_cap) {
     HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_BIGTILES - C_WORDS + 1), c->stream));
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(sgns_enqueue(c, d_off, d_tok, dbg, c->ev1));
+    HIP_TRY(sgns_enqueue_into(c, d_off, d_tok, dbg, c->ev1, tiles, tiles_cap));
     HIP_TRY(hipEventRecord(c->ev2, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev2));
     HIP_TRY(hipEventElapsedTime(&ms[0], c->ev0, c->ev2));

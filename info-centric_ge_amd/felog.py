@@ -343,12 +343,13 @@ class Felog:
                     "felog_corpus_copy_out")
         return off, tok[:c.n_tokens], wid[:c.n_walks]
 
-    def csr(self):
+    def csr(self, weights=True):
+        """row_ptr, col and (unless weights=False) w of the CSR."""
         rp = np.empty(self.V + 1, np.uint64)
         col = np.empty(max(self.E, 1), np.uint32)
-        w = np.empty(max(self.E, 1), np.float32)
+        w = np.empty(max(self.E, 1), np.float32) if weights else None
         self._check(self.lib.felog_csr_copy_out(self.ctx, _ptr(rp), _ptr(col), _ptr(w)), "felog_csr_copy_out")
-        return rp, col[:self.E], w[:self.E]
+        return rp, col[:self.E], (w[:self.E] if weights else None)
 
     def alias(self):
         t = np.empty(max(self.E, 1), np.uint32)
