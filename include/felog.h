@@ -180,6 +180,17 @@ felog_status felog_train_round(felog_ctx *ctx, const felog_corpus *corpus);
 felog_status felog_train_corpus(felog_ctx *ctx, const uint64_t *offsets,
                                 const uint32_t *tokens, uint64_t n_sentences,
                                 uint32_t round);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): felog_train_corpus with the staged corpus and the SGNS
+ * tile descriptors in a static 64 MiB device array of the library; the call
+ * makes no device allocation.  A corpus that needs more (8 B per sentence +
+ * 4 B per token + 640 B per tile, each part 256-byte aligned) returns
+ * FELOG_ERR_CAPACITY.  Every context on a GPU shares that GPU's array:
+ * calls that overlap on one GPU race and must be serialised. */
+felog_status felog_train_corpus_noalloc(felog_ctx *ctx, const uint64_t *offsets,
+                                        const uint32_t *tokens,
+                                        uint64_t n_sentences, uint32_t round);
 /* Convergence evaluation (src/word2vec.cu:2016-2080): flags vertices whose
  * mean cosine to <= 30 neighbours exceeds 0.65; *stop = 1 when the
  * unconverged count stopped shrinking (ratio > 0.7). */

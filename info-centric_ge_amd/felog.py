@@ -113,6 +113,7 @@ def load_library(path=LIB_PATH):
         "felog_walk_round": (C.c_int32, [P, C.c_uint32, C.POINTER(Corpus)]),
         "felog_train_round": (C.c_int32, [P, C.POINTER(Corpus)]),
         "felog_train_corpus": (C.c_int32, [P, P, P, C.c_uint64, C.c_uint32]),
+        "felog_train_corpus_noalloc": (C.c_int32, [P, P, P, C.c_uint64, C.c_uint32]),
         "felog_evaluate": (C.c_int32, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_int)]),
         "felog_run": (C.c_int32, [P]),
         "felog_corpus_copy_out": (C.c_int32, [P, P, P, P]),
@@ -239,6 +240,14 @@ class Felog:
         tokens = np.ascontiguousarray(tokens, dtype=np.uint32)
         self._check(self.lib.felog_train_corpus(self.ctx, _ptr(offsets), _ptr(tokens), len(offsets) - 1, rnd),
                     "felog_train_corpus")
+
+    def train_corpus_noalloc(self, offsets, tokens, rnd):
+        """train_corpus without device allocation (the library's static
+        scratch); a corpus beyond it raises with status ERR_CAPACITY."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        tokens = np.ascontiguousarray(tokens, dtype=np.uint32)
+        self._check(self.lib.felog_train_corpus_noalloc(self.ctx, _ptr(offsets), _ptr(tokens), len(offsets) - 1,
+                                                        rnd), "felog_train_corpus_noalloc")
 
     def evaluate(self):
         nu = C.c_uint32()
