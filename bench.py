@@ -30,6 +30,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -131,55 +132,89 @@ def end_to_end(src, dst, w, n, args, model):
     else:
         ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
     out = {}
-    for form, prec, budgeted in (("plain_fp32", felog.FP32, False), ("bf16_surrogate", felog.BF16, True)):
-        f = felog.Felog(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20, min_R=2,
-                        precision=prec, seed=args.seed)
+    forms = (("plain_fp32", felog.FP32, None), ("bf16_surrogate", felog.BF16, model))
+    if args.e2e_ablation:
+        # the same budgeted run with the network replaced by one constant
+        # budget (count min_R, length min_L + 2): what the network itself buys
+        import walk_budget
+        forms += (("bf16_constant_budget", felog.BF16, walk_budget.constant_model(2, 22)),)
+    for form, prec, budget_model in forms:
+        opts = dict(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20, min_R=2,
+                    precision=prec, seed=args.seed)
+        # tile shape for this precision, timed on a first-round corpus of a
+        # separate context (a one-time calibration, not part of the run)
+        f = felog.Felog(**opts)
         f.set_edges(ts, td, tw, n, make_undirected=True)
         f.build()
-        # tile shape for this precision, timed on a first-round corpus
         f.walk_round(1)
         f.tune_sgns(sample=0, reps=1)
         shape = f.tile_shape()
-        f.visits_reset()
+        f.close()
+        f = felog.Felog(**opts)
+        f.set_edges(ts, td, tw, n, make_undirected=True)  # host edge list to the device: input I/O, not timed
+        f.set_tile_shape(shape["rows"], shape["waves_per_simd"], shape["groups_per_cu"])
         extra = {}
-        if budgeted:
-            f.budget_set(model)
+        # timed: CSR + alias + vocabulary + tables, the budget's prediction,
+        # every round to the stop
+        t0 = time.perf_counter()
+        f.build()
+        t_build = time.perf_counter() - t0
+        f.visits_reset()
+        if budget_model is not None:
+            f.budget_set(budget_model)
             # one convergence test at min_R for vertices whose budget ends there
             f.budget_recheck(True)
+        t1 = time.perf_counter()
+        f.run_graph()  # felog_run_budget / felog_run as captured rounds
+        t2 = time.perf_counter()
+        if budget_model is not None:
             cnt, ln, rounds = f.budget()
             extra = {"budgeted_vertices": int((cnt > 0).sum()), "plain_rule_fallback": int((cnt == 0).sum()),
-                     "max_count": int(rounds), "recheck": True}
-        t0 = time.perf_counter()
-        f.run_graph()  # felog_run_budget / felog_run as captured rounds
-        wall = time.perf_counter() - t0
-        if budgeted:
-            extra["recheck_extended"] = int((f.budget()[0] > cnt).sum())
+                     "max_count": int(rounds), "recheck": True, "budget_set_ms": (t1 - t0 - t_build) * 1e3}
         st = f.stats()
         vs = f.visit_stats()
         auc = None
         if pos is not None:
             emb, _ = f.embeddings()
             auc = _auc(emb, pos, neg, orc)
-        print(f"[bench] e2e {form}: {st['rounds_done']} rounds, {wall * 1e3:.1f} ms", file=sys.stderr, flush=True)
-        out[form] = dict(wall_ms=wall * 1e3, rounds=st["rounds_done"], tokens=vs["visits"],
-                         tokens_per_s=vs["visits"] / wall, auc=auc, visit_kl=vs["kl"],
-                         visit_entropy=vs["entropy"], rounds_gpu_ms=st["graph_ms_total"],
+        print(f"[bench] e2e {form}: build {t_build * 1e3:.1f} ms + {st['rounds_done']} rounds "
+              f"{(t2 - t1) * 1e3:.1f} ms, AUC {auc}", file=sys.stderr, flush=True)
+        out[form] = dict(wall_ms=(t2 - t0) * 1e3, build_ms=t_build * 1e3, rounds_ms=(t2 - t1) * 1e3,
+                         rounds=st["rounds_done"], tokens=vs["visits"], tokens_per_s=vs["visits"] / (t2 - t0),
+                         auc=auc, visit_kl=vs["kl"], visit_entropy=vs["entropy"],
+                         rounds_gpu_ms=st["graph_ms_total"],
                          tile_shape={k: shape[k] for k in ("rows", "waves_per_simd", "groups_per_cu")}, **extra)
         f.close()
     p, b = out["plain_fp32"], out["bf16_surrogate"]
     out["throughput_ratio"] = p["wall_ms"] / b["wall_ms"]
+    out["throughput_ratio_rounds_only"] = p["rounds_ms"] / b["rounds_ms"]
     out["auc_gap"] = None if p["auc"] is None else b["auc"] - p["auc"]
     out["definition"] = ("throughput_ratio = wall time of the plain FP32 run / wall time of the BF16 + "
-                         "learned-budget run, each from the first walk to its stop on the same graph "
-                         "(vertices embedded per second); the budgeted run scores the vertices whose "
-                         "budget ends at min_R once and walks those not converged one more round; "
-                         "auc_gap = held-out AUC difference")
+                         "learned-budget run, each from felog_build (CSR, alias, vocabulary, tables; the "
+                         "budget's prediction) to its stop on the same graph, graph input excluded; the "
+                         "budgeted run scores the vertices whose budget ends at min_R once and walks those "
+                         "not converged one more round; auc_gap = held-out AUC difference")
     return out
+
+
+def cpu_model():
+    """CPU model, sockets and logical CPUs of this host (from lscpu)."""
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"), "cpus": info.get("CPU(s)"),
+            "threads_per_core": info.get("Thread(s) per core")}
 
 
 def cpu_baseline(src, dst, w, n, args, seed, threads):
     """The oracle (CPU restatement) timed on the host cores over a bounded
-    sample of walkers: walks, then word2vec skip-gram on their corpus."""
+    sample of walkers: walks, then word2vec skip-gram on their corpus; the
+    median of three runs of each."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     rp, col, wo = orc.build_csr(n, src, dst, w, True)
@@ -187,16 +222,20 @@ def cpu_baseline(src, dst, w, n, args, seed, threads):
     deg = np.diff(rp.astype(np.int64)).astype(np.uint32)
     order, rank, inv, vs, tw = orc.vocab(deg, 5)
     sample = min(n, args.cpu_walkers)
-    t0 = time.perf_counter()
-    off, tok, wid, steps = orc.walks(n, rp, col, th, al, None, sample, 20, 1000, 1, seed, threads=threads)
-    t_walk = time.perf_counter() - t0
-    rng = np.random.default_rng(0)
-    syn0 = ((rng.random((n, args.dim)) - 0.5) / args.dim).astype(np.float32)
-    syn1 = np.zeros((n, args.dim), np.float32)
-    t0 = time.perf_counter()
-    words, pairs = orc.w2v_train(syn0, syn1, off, tok, deg, order, rank, inv, vs, tw, window=10, negative=5,
-                                 sample=1e-3, alpha=0.025, use_lut=True, threads=threads)
-    t_train = time.perf_counter() - t0
+    t_walks, t_trains = [], []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        off, tok, wid, steps = orc.walks(n, rp, col, th, al, None, sample, 20, 1000, 1, seed, threads=threads)
+        t_walks.append(time.perf_counter() - t0)
+        rng = np.random.default_rng(0)
+        syn0 = ((rng.random((n, args.dim)) - 0.5) / args.dim).astype(np.float32)
+        syn1 = np.zeros((n, args.dim), np.float32)
+        t0 = time.perf_counter()
+        words, pairs = orc.w2v_train(syn0, syn1, off, tok, deg, order, rank, inv, vs, tw, window=10, negative=5,
+                                     sample=1e-3, alpha=0.025, use_lut=True, threads=threads)
+        t_trains.append(time.perf_counter() - t0)
+        del syn0, syn1
+    t_walk, t_train = float(np.median(t_walks)), float(np.median(t_trains))
     tokens = int(off[-1])
     return {
         "value": tokens / (t_walk + t_train),
@@ -204,9 +243,11 @@ def cpu_baseline(src, dst, w, n, args, seed, threads):
         "cores": threads,
         "kind": "port",
         "sample": f"{sample} walkers of the same graph ({tokens} tokens); walk {steps / t_walk:.4g} steps/s, "
-                  f"word2vec skip-gram {words / t_train:.4g} words/s",
+                  f"word2vec skip-gram {words / t_train:.4g} words/s; median of 3 runs",
         "walk_steps_per_s": steps / t_walk,
         "sgns_words_per_s": words / t_train,
+        "runs_s": {"walk": t_walks, "word2vec": t_trains},
+        "host": cpu_model(),
     }
 
 
@@ -223,6 +264,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end plain vs BF16 + budget runs")
     ap.add_argument("--e2e-no-auc", action="store_true", help="end-to-end runs on the whole graph, no AUC")
+    ap.add_argument("--e2e-ablation", action="store_true",
+                    help="also run the budgeted form with a constant budget instead of the network")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tile shape (profiling runs: one tile kernel variant)")
     args = ap.parse_args()
