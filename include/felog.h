@@ -95,7 +95,20 @@ typedef struct felog_options {
                                   step there and back; common neighbours =
                                   (int) edge data (include/walk.hpp:976-1019).
                                   0: weight-proportional alias step          */
+    int32_t trainer;           /* FELOG_TRAIN_* (below)                      */
+    uint32_t w2v_table_size;   /* unigram table entries of the per-pair
+                                  trainers (0 -> 1e8, src/word2vec.cu:171)   */
 } felog_options;
+
+/* The trainer of felog_train_round / felog_train_corpus / the captured round.
+ * HOGBATCH (the default): the MFMA tile kernel (shared negatives per group of
+ *   centres, tile-summed updates; include/dsgl.hpp:832-1407 is its model),
+ *   FP32 or BF16 tables; replaces skip_gram_kernel (src/word2vec.cu:442-536).
+ * SKIPGRAM, SKIPGRAM_REUSE, CBOW: the reference's per-pair kernels with its
+ *   update order and random streams - skip_gram_kernel, __sgNegReuse
+ *   (src/word2vec.cu:368-440) and cbow_kernel (:538-646) - on FP32 tables,
+ *   with the 1e8-entry unigram table and the sigmoid table. */
+enum { FELOG_TRAIN_HOGBATCH = 0, FELOG_TRAIN_SKIPGRAM = 1, FELOG_TRAIN_SKIPGRAM_REUSE = 2, FELOG_TRAIN_CBOW = 3 };
 
 /* How SGNS tiles write their row deltas (Hogwild either way).
  * WRITEBACK (the default): each tile stores staged-row + delta (bf16:
@@ -251,6 +264,9 @@ felog_status felog_unigram_copy_out(felog_ctx *ctx, uint32_t *thresh,
                                     uint32_t *alias_idx);
 felog_status felog_flags_copy_out(felog_ctx *ctx, uint8_t *flags,
                                   float *scores);
+/* The per-pair trainers' unigram table (w2v_table_size vocabulary indices,
+ * src/word2vec.cu:674-694); built on first use. */
+felog_status felog_w2v_table_copy_out(felog_ctx *ctx, int32_t *table);
 /* Rows indexed by vertex id, widened to f32; either pointer may be NULL. */
 felog_status felog_embeddings_copy_out(felog_ctx *ctx, float *syn0,
                                        float *syn1neg);

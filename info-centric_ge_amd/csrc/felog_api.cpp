@@ -71,7 +71,8 @@ void free_all(felog_ctx *c) {
                     c->d_keep_thr, c->d_T, c->d_dT, c->d_syn0, c->d_flags,
                     c->d_scores, c->d_active, c->d_sel, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
-                    c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats};
+                    c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats,
+                    c->d_w2v_ftok, c->d_w2v_kcnt, c->d_w2v_blk, c->d_w2v_table, c->d_w2v_exp};
     for (void *p : ptrs) dfree(p);
     drop_round_graph(c);
     dfree(c->d_rs);
@@ -131,6 +132,9 @@ felog_status felog_create(felog_ctx **out, const felog_options *opt) {
     else if (o.max_walk_len < 2 || o.max_walk_len > (1u << 20)) bad = "max_walk_len must be in [2, 2^20]";
     else if (!(o.alpha > 0.f)) bad = "-alpha must be > 0";
     else if (o.update_mode < FELOG_UPDATE_DEFAULT || o.update_mode > FELOG_UPDATE_WRITEBACK) bad = "bad update_mode";
+    else if (o.trainer < FELOG_TRAIN_HOGBATCH || o.trainer > FELOG_TRAIN_CBOW) bad = "bad trainer";
+    else if (o.trainer != FELOG_TRAIN_HOGBATCH && o.precision != FELOG_FP32)
+        bad = "the per-pair trainers update FP32 tables";
     if (bad.empty() && sgns_group_centres(o.batch_size, o.window, o.negative) < 1)
         bad = "-window/-negative leave no room for a centre in an SGNS group";
     if (!bad.empty()) {
@@ -336,7 +340,8 @@ static felog_status train_device_corpus(felog_ctx *c, const uint64_t *d_off, con
     if (n >= 0xFFFFFFFFull) return fail(c, FELOG_ERR_ARG, "a corpus holds fewer than 2^32 - 1 sentences");
     float ms[3] = {0.f, 0.f, 0.f};
     TRY(c, round_set(c, round, alpha, (uint32_t)n));
-    TRY(c, run_sgns(c, d_off, d_tok, n, n_tok, nullptr, ms));
+    if (c->opt.trainer != FELOG_TRAIN_HOGBATCH) TRY(c, run_w2v(c, d_off, d_tok, n, n_tok, ms));
+    else TRY(c, run_sgns(c, d_off, d_tok, n, n_tok, nullptr, ms));
     c->st.train_words = c->h_counters[C_WORDS];
     c->st.train_kept = c->h_counters[C_KEPT];
     c->st.train_pairs = c->h_counters[C_PAIRS];
@@ -345,7 +350,7 @@ static felog_status train_device_corpus(felog_ctx *c, const uint64_t *d_off, con
     c->st.sgns_prep_ms = ms[1];
     c->st.sgns_tile_ms = ms[2];
     c->st.train_ms_total += ms[0];
-    if (n + c->h_counters[C_TILES] > c->tiles_cap)
+    if (c->opt.trainer == FELOG_TRAIN_HOGBATCH && n + c->h_counters[C_TILES] > c->tiles_cap)
         return fail(c, FELOG_ERR_INVARIANT, "SGNS tile descriptors exceeded their bound");
     return FELOG_OK;
 }
@@ -398,6 +403,7 @@ felog_status felog_debug_sgns(felog_ctx *c, const uint64_t *offsets, const uint3
     if (s) return s;
     if (!offsets || !n_groups || !groups_out || !rows_out || !logits_out || !dv_out || !du_out || max_groups == 0)
         return fail(c, FELOG_ERR_ARG, "null debug buffer");
+    if (c->opt.trainer != FELOG_TRAIN_HOGBATCH) return fail(c, FELOG_ERR_STATE, "the tile trainer is not selected");
     if (n >= 0xFFFFFFFFull) return fail(c, FELOG_ERR_ARG, "a corpus holds fewer than 2^32 - 1 sentences");
     const uint64_t nt = offsets[n];
     for (uint64_t i = 0; i < nt; ++i)
@@ -696,7 +702,9 @@ hipError_t capture_round(felog_ctx *c, bool visits) {
     if (e == hipSuccess) e = walk_enqueue(c);
     if (e == hipSuccess) e = compact_enqueue(c);
     if (e == hipSuccess && visits) e = add_round_visits(c);
-    if (e == hipSuccess) e = sgns_enqueue(c, c->d_off, c->d_tok, nullptr, nullptr);
+    if (e == hipSuccess)
+        e = c->opt.trainer != FELOG_TRAIN_HOGBATCH ? w2v_enqueue(c, c->d_off, c->d_tok)
+                                                   : sgns_enqueue(c, c->d_off, c->d_tok, nullptr, nullptr);
     if (e == hipSuccess) e = eval_enqueue(c, true);
     if (e == hipSuccess && c->budget_on && c->budget_recheck) e = recheck_enqueue(c, true);
     if (e == hipSuccess) e = round_end(c);
@@ -716,8 +724,9 @@ felog_status felog_run_graph(felog_ctx *c, uint32_t max_rounds) {
     if (s) return s;
     const bool budget = c->budget_on;
     const bool visits = c->d_visits != nullptr;
-    // descriptors for the largest corpus a round can write
-    TRY(c, sgns_reserve(c, c->walkers, c->tok_cap));
+    // buffers for the largest corpus a round can write
+    if (c->opt.trainer != FELOG_TRAIN_HOGBATCH) TRY(c, w2v_reserve(c, c->walkers, c->tok_cap));
+    else TRY(c, sgns_reserve(c, c->walkers, c->tok_cap));
     if (c->round_exec && (c->round_exec_budget != budget || c->round_exec_visits != visits)) drop_round_graph(c);
     if (!c->round_exec) {
         TRY(c, capture_round(c, visits));
@@ -879,6 +888,7 @@ felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t r
     if (s) return s;
     if (!c->corpus.valid || !c->corpus.from_walk)
         return fail(c, FELOG_ERR_STATE, "no walk corpus: call felog_walk_round first");
+    if (c->opt.trainer != FELOG_TRAIN_HOGBATCH) return fail(c, FELOG_ERR_STATE, "the tile trainer is not selected");
     if (reps == 0) reps = 1;
     uint64_t n = c->corpus.n_walks;
     if (sample_sentences && sample_sentences < n) n = sample_sentences;
@@ -1007,6 +1017,15 @@ felog_status felog_flags_copy_out(felog_ctx *c, uint8_t *flags, float *scores) {
     if (s) return s;
     if (flags) TRY(c, hipMemcpy(flags, c->d_flags, c->V, hipMemcpyDeviceToHost));
     if (scores) TRY(c, hipMemcpy(scores, c->d_scores, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+felog_status felog_w2v_table_copy_out(felog_ctx *c, int32_t *table) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!table) return fail(c, FELOG_ERR_ARG, "null table");
+    TRY(c, w2v_prepare(c));
+    TRY(c, hipMemcpy(table, c->d_w2v_table, (uint64_t)c->w2v_table_size * 4, hipMemcpyDeviceToHost));
     return FELOG_OK;
 }
 

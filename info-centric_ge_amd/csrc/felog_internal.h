@@ -164,6 +164,14 @@ struct felog_ctx {
     uint64_t visits_total = 0;
     double *d_vstats = nullptr;          // 3 doubles
 
+    // per-pair word2vec trainers: filtered corpus, batch indices, the
+    // unigram table and the sigmoid table
+    uint32_t *d_w2v_ftok = nullptr, *d_w2v_kcnt = nullptr, *d_w2v_blk = nullptr;
+    uint64_t w2v_cap_sent = 0, w2v_cap_tok = 0;
+    int32_t *d_w2v_table = nullptr;
+    uint32_t w2v_table_size = 0;
+    float *d_w2v_exp = nullptr;
+
     felog_stats st{};
 };
 
@@ -238,6 +246,12 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
                              hipEvent_t mid, void *tiles, uint64_t tiles_cap);
 hipError_t run_sgns_into(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
                          float *ms, void *tiles, uint64_t tiles_cap);
+// w2v_kernels.hip: the reference's per-pair trainers (opt.trainer != 0)
+hipError_t w2v_prepare(felog_ctx *c);
+hipError_t w2v_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok);
+hipError_t w2v_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok);
+hipError_t run_w2v(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
+                   float *ms);
 // eval_kernel.hip
 // gated: evaluate only when the round state says so (captured rounds)
 hipError_t eval_enqueue(felog_ctx *c, bool gated);

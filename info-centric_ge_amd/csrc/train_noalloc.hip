@@ -67,6 +67,7 @@ extern "C" felog_status felog_train_corpus_noalloc(felog_ctx *c, const uint64_t 
         c->err = std::string(what) + ": " + hipGetErrorString(e);
         return FELOG_ERR_HIP;
     };
+    if (c->opt.trainer != FELOG_TRAIN_HOGBATCH) return fail(FELOG_ERR_STATE, "the tile trainer is not selected");
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     if (round == 0) return fail(FELOG_ERR_ARG, "rounds are numbered from 1");

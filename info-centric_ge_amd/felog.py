@@ -23,6 +23,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "felog.h")
 
 FP32, BF16 = 0, 1
 UPDATE_DEFAULT, UPDATE_ATOMIC, UPDATE_WRITEBACK = 0, 1, 2
+TRAIN_HOGBATCH, TRAIN_SKIPGRAM, TRAIN_SKIPGRAM_REUSE, TRAIN_CBOW = 0, 1, 2, 3
 STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_IO", -3: "ERR_HIP", -4: "ERR_OOM",
           -5: "ERR_INVARIANT", -6: "ERR_STATE", -7: "ERR_CAPACITY"}
 
@@ -43,6 +44,7 @@ class Options(C.Structure):
         ("max_walk_len", C.c_uint32), ("overflow_capacity", C.c_uint64), ("seed", C.c_uint64),
         ("compat_lr", C.c_int32), ("compat_vocab_break", C.c_int32), ("max_rounds", C.c_uint32),
         ("device", C.c_int32), ("update_mode", C.c_int32), ("compat_walk", C.c_int32),
+        ("trainer", C.c_int32), ("w2v_table_size", C.c_uint32),
     ]
 
 
@@ -122,6 +124,7 @@ def load_library(path=LIB_PATH):
         "felog_vocab_copy_out": (C.c_int32, [P, P, P]),
         "felog_unigram_copy_out": (C.c_int32, [P, P, P]),
         "felog_flags_copy_out": (C.c_int32, [P, P, P]),
+        "felog_w2v_table_copy_out": (C.c_int32, [P, P]),
         "felog_embeddings_copy_out": (C.c_int32, [P, P, P]),
         "felog_embeddings_copy_in": (C.c_int32, [P, P, P]),
         "felog_write_walks": (C.c_int32, [P, C.c_char_p]),
@@ -383,6 +386,13 @@ class Felog:
         s = np.empty(self.V, np.float32)
         self._check(self.lib.felog_flags_copy_out(self.ctx, _ptr(f), _ptr(s)), "felog_flags_copy_out")
         return f, s
+
+    def w2v_table(self):
+        """The per-pair trainers' unigram table (vocabulary indices)."""
+        n = self.opt.w2v_table_size or 100_000_000
+        t = np.empty(n, np.int32)
+        self._check(self.lib.felog_w2v_table_copy_out(self.ctx, _ptr(t)), "felog_w2v_table_copy_out")
+        return t
 
     def embeddings(self):
         d = self.opt.size

@@ -12,7 +12,16 @@
 //      (ArgPos, src/word2vec.cu:2182-2192,2269-2286).
 // Switches of this build (also found anywhere): --precision fp32|bf16,
 // --seed N, --max-rounds N, --max-walk-len N, --device N, --write-walks,
-// --felog-compat-lr, --felog-compat-vocab, --felog-compat-walk.
+// --felog-compat-lr, --felog-compat-vocab, --felog-compat-walk,
+// --felog-compat-sgns.
+// Trainers, as the reference's -cbow / -reuse-neg pick its kernels
+// (src/word2vec.cu:163, 931-1007): -cbow 1 (the default) trains cbow_kernel's
+// rule, -cbow 0 -reuse-neg 1 (the default) __sgNegReuse's, -cbow 0
+// -reuse-neg 0 the MFMA HogBatch tile trainer that replaces skip_gram_kernel
+// (--felog-compat-sgns: skip_gram_kernel's per-pair rule instead).  The
+// per-pair trainers follow the reference's step ExponentialDecayLR(0.025,
+// 0.1) and ignore -alpha, as the reference does (src/word2vec.cu:1958,2001);
+// the tile trainer honours -alpha (default 0.0025, its stable step).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -88,7 +97,10 @@ int main(int argc, char **argv) {
     if (f.g.empty() || f.v <= 0 || f.w < 0 || f.min_L < 0 || f.min_R < 0) {
         std::fprintf(stderr,
                      "usage: felog -g <graph.bin> -v <vertices> -w <walkers> --min_L <n> --min_R <n> "
-                     "[--make-undirected] [-o <prefix>] -emb_output <file> [word2vec flags]\n");
+                     "[--make-undirected] [-o <prefix>] -emb_output <file> [word2vec flags]\n"
+                     "  -cbow 1 (default): CBOW; -cbow 0 -reuse-neg 1 (default): skip-gram with per-sentence\n"
+                     "  negatives; -cbow 0 -reuse-neg 0: the HogBatch tile trainer (-alpha honoured, default\n"
+                     "  0.0025); the per-pair trainers use the reference's step 0.025 e^{-0.1 r} and ignore -alpha\n");
         return 1;
     }
     felog_options o;
@@ -137,7 +149,21 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "felog: -hs and -classes are not part of this build\n");
         return 1;
     }
-    if (cbow) std::printf("[ Warning ] CBOW is not part of this build; training skip-gram (-cbow 0)\n");
+    int reuse_neg = 1;  // src/word2vec.cu:163
+    if ((i = arg_pos("-reuse-neg", argc, argv, true)) > 0) reuse_neg = std::atoi(argv[i + 1]);
+    const bool compat_sgns = arg_pos("--felog-compat-sgns", argc, argv, false) > 0;
+    o.trainer = cbow ? FELOG_TRAIN_CBOW
+                     : reuse_neg ? FELOG_TRAIN_SKIPGRAM_REUSE
+                                 : compat_sgns ? FELOG_TRAIN_SKIPGRAM : FELOG_TRAIN_HOGBATCH;
+    if (o.trainer != FELOG_TRAIN_HOGBATCH) {
+        o.compat_lr = 1;
+        if (o.precision != FELOG_FP32) {
+            std::fprintf(stderr, "felog: the CBOW / negative-reuse / per-pair skip-gram trainers use FP32 tables\n");
+            return 1;
+        }
+    }
+    static const char *kTrainer[] = {"HogBatch tile (skip-gram)", "skip-gram, per pair", "skip-gram, per-sentence negatives", "CBOW"};
+    std::printf("[ 0 ] trainer: %s\n", kTrainer[o.trainer]);
     if (write_walks && f.o.empty()) {
         std::fprintf(stderr, "felog: --write-walks needs -o <prefix>\n");
         return 1;

@@ -693,11 +693,31 @@ uint32_t orc_w2v_negative(uint64_t *next_random, const int32_t *table, uint32_t 
     return tr;
 }
 
+#define D_W2V 0x70000000u
+
+static float w2v_grad(float f, float label, float alpha, int use_lut, const float *expTable) {
+    if (f > MAX_EXP) return (label - 1) * alpha;
+    if (f < -MAX_EXP) return (label - 0) * alpha;
+    if (use_lut) return (label - expTable[orc_w2v_lut_index(f)]) * alpha;
+    return (label - 1.f / (1.f + expf(-f))) * alpha;
+}
+
 uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const uint64_t *offsets, const uint32_t *tokens,
                        uint64_t n_sent, const uint32_t *deg, const uint32_t *order, const uint32_t *rank,
                        const uint8_t *in_vocab, uint64_t vocab_size, uint64_t train_words, uint32_t window,
                        uint32_t negative, float sample, float alpha, int use_lut, int compat_break,
                        uint32_t table_size, int threads, uint64_t *pairs_out) {
+    return orc_w2v_train_mode(V, d, syn0, syn1, offsets, tokens, n_sent, deg, order, rank, in_vocab, vocab_size,
+                              train_words, window, negative, sample, alpha, use_lut, compat_break, table_size, threads,
+                              pairs_out, 0, 0, 0);
+}
+
+uint64_t orc_w2v_train_mode(uint32_t V, uint32_t d, float *syn0, float *syn1, const uint64_t *offsets,
+                            const uint32_t *tokens, uint64_t n_sent, const uint32_t *deg, const uint32_t *order,
+                            const uint32_t *rank, const uint8_t *in_vocab, uint64_t vocab_size, uint64_t train_words,
+                            uint32_t window, uint32_t negative, float sample, float alpha, int use_lut,
+                            int compat_break, uint32_t table_size, int threads, uint64_t *pairs_out, int mode,
+                            uint64_t seed, uint32_t round) {
     /* expTable, src/word2vec.cu:2292-2297 */
     float expTable[EXP_TABLE_SIZE + 1];
     orc_w2v_exp_table(expTable);
@@ -766,22 +786,49 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
 #pragma omp parallel reduction(+ : pairs)
     {
         float *neu1e = (float *)malloc((uint64_t)d * sizeof(float));
+        float *neu1 = (float *)malloc((uint64_t)d * sizeof(float));
+        uint32_t negs[64];
 #pragma omp for schedule(dynamic, 64)
         for (int64_t s = 0; s < (int64_t)n_sent; ++s) {
             const uint64_t ss = soff[s], se = soff[s + 1];
             unsigned long long next_random = (unsigned long long)blk[s];
+            if (mode == 1) {
+                /* __sgNegReuse (src/word2vec.cu:368-440): one set of negatives
+                 * per sentence, drawn from the unigram table; here the draw is
+                 * Philox(seed; j, W2V, sentence, round) word 0 in place of the
+                 * host LCG of :1251-1259 */
+                for (uint32_t j = 0; j < negative && j < 64; ++j) {
+                    uint32_t r[4];
+                    rnd(seed, j, D_W2V, (uint32_t)s, round, r);
+                    uint32_t tr = (uint32_t)table[r[0] % table_size];
+                    if (tr == 0) tr = (uint32_t)(r[0] % (vocab_size - 1) + 1);
+                    negs[j] = order[tr];
+                }
+            }
             for (uint64_t pos = ss; pos < se; ++pos) {
                 const uint32_t word = sen[pos];
-                next_random = next_random * 2514903917ull + 11;  /* src/word2vec.cu:458 (sic) */
-                const uint32_t b = (uint32_t)(next_random % window);
-                for (uint32_t a = b; a < window * 2 + 1 - b; ++a) {
-                    if (a == window) continue;
-                    const int64_t c = (int64_t)pos - window + a;
-                    if (c < (int64_t)ss || c >= (int64_t)se) continue;
-                    const uint32_t last = sen[c];
-                    float *l1 = syn0 + (uint64_t)last * d;
+                uint32_t b = 0;
+                if (mode != 1) {
+                    next_random = next_random * 2514903917ull + 11;  /* src/word2vec.cu:458,555 (sic) */
+                    b = (uint32_t)(next_random % window);
+                }
+                if (mode == 2) {
+                    /* cbow_kernel (src/word2vec.cu:538-646): the mean of the
+                     * context rows predicts the centre */
+                    uint32_t cw = 0;
+                    memset(neu1, 0, (uint64_t)d * sizeof(float));
+                    for (uint32_t a = b; a < window * 2 + 1 - b; ++a) {
+                        if (a == window) continue;
+                        const int64_t c = (int64_t)pos - window + a;
+                        if (c < (int64_t)ss || c >= (int64_t)se) continue;
+                        const float *l1 = syn0 + (uint64_t)sen[c] * d;
+                        for (uint32_t e = 0; e < d; ++e) neu1[e] += l1[e];
+                        ++cw;
+                    }
+                    if (!cw) continue;
+                    for (uint32_t e = 0; e < d; ++e) neu1[e] /= (float)cw;
                     memset(neu1e, 0, (uint64_t)d * sizeof(float));
-                    ++pairs;
+                    pairs += cw;
                     for (uint32_t dd = 0; dd < negative + 1; ++dd) {
                         uint32_t target;
                         float label;
@@ -798,13 +845,50 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                         }
                         float *l2 = syn1 + (uint64_t)target * d;
                         float f = 0.f;
-#pragma omp simd reduction(+ : f)
+                        for (uint32_t e = 0; e < d; ++e) f += neu1[e] * l2[e];
+                        const float g = w2v_grad(f, label, alpha, use_lut, expTable);
+                        for (uint32_t e = 0; e < d; ++e) neu1e[e] += g * l2[e];
+                        for (uint32_t e = 0; e < d; ++e) l2[e] += g * neu1[e];
+                    }
+                    for (uint32_t a = b; a < window * 2 + 1 - b; ++a) {
+                        if (a == window) continue;
+                        const int64_t c = (int64_t)pos - window + a;
+                        if (c < (int64_t)ss || c >= (int64_t)se) continue;
+                        float *l1 = syn0 + (uint64_t)sen[c] * d;
+                        for (uint32_t e = 0; e < d; ++e) l1[e] += neu1e[e];
+                    }
+                    continue;
+                }
+                for (uint32_t a = b; a < window * 2 + 1 - b; ++a) {
+                    if (a == window) continue;
+                    const int64_t c = (int64_t)pos - window + a;
+                    if (c < (int64_t)ss || c >= (int64_t)se) continue;
+                    const uint32_t last = sen[c];
+                    float *l1 = syn0 + (uint64_t)last * d;
+                    memset(neu1e, 0, (uint64_t)d * sizeof(float));
+                    ++pairs;
+                    for (uint32_t dd = 0; dd < negative + 1; ++dd) {
+                        uint32_t target;
+                        float label;
+                        if (dd == 0) {
+                            target = word;
+                            label = 1.f;
+                        } else if (mode == 1) {
+                            target = negs[dd - 1];
+                            if (target == word) continue;
+                            label = 0.f;
+                        } else {
+                            uint64_t nr = next_random;
+                            const uint32_t tr = orc_w2v_negative(&nr, table, table_size, vocab_size);
+                            next_random = nr;
+                            target = order[tr];
+                            if (target == word) continue;
+                            label = 0.f;
+                        }
+                        float *l2 = syn1 + (uint64_t)target * d;
+                        float f = 0.f;
                         for (uint32_t e = 0; e < d; ++e) f += l1[e] * l2[e];
-                        float g;
-                        if (f > MAX_EXP) g = (label - 1) * alpha;
-                        else if (f < -MAX_EXP) g = (label - 0) * alpha;
-                        else if (use_lut) g = (label - expTable[orc_w2v_lut_index(f)]) * alpha;
-                        else g = (label - 1.f / (1.f + expf(-f))) * alpha;
+                        const float g = w2v_grad(f, label, alpha, use_lut, expTable);
                         for (uint32_t e = 0; e < d; ++e) neu1e[e] += g * l2[e];
                         for (uint32_t e = 0; e < d; ++e) l2[e] += g * l1[e];
                     }
@@ -813,6 +897,7 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
             }
         }
         free(neu1e);
+        free(neu1);
     }
     if (pairs_out) *pairs_out = pairs;
     free(table); free(thr16); free(sen); free(soff); free(blk);

@@ -122,6 +122,18 @@ uint64_t orc_w2v_train(uint32_t V, uint32_t d, float *syn0, float *syn1, const u
                        uint64_t train_words, uint32_t window, uint32_t negative, float sample, float alpha,
                        int use_lut, int compat_break, uint32_t table_size, int threads, uint64_t *pairs);
 
+/* The same with the trainer chosen by mode: 0 skip_gram_kernel, 1
+ * __sgNegReuse (src/word2vec.cu:368-440: a full window and one set of
+ * negatives per sentence, drawn as Philox(seed; j, W2V, sentence, round)
+ * word 0 through the unigram table), 2 cbow_kernel (src/word2vec.cu:538-646).
+ * pairs counts (centre, context) pairs, for CBOW the context rows averaged. */
+uint64_t orc_w2v_train_mode(uint32_t V, uint32_t d, float *syn0, float *syn1, const uint64_t *offsets,
+                            const uint32_t *tokens, uint64_t n_sent, const uint32_t *deg, const uint32_t *order,
+                            const uint32_t *rank, const uint8_t *in_vocab, uint64_t vocab_size, uint64_t train_words,
+                            uint32_t window, uint32_t negative, float sample, float alpha, int use_lut,
+                            int compat_break, uint32_t table_size, int threads, uint64_t *pairs, int mode,
+                            uint64_t seed, uint32_t round);
+
 /* word2vec's unigram table over vocabulary indices (src/word2vec.cu:674-694);
  * cn[] = counts in vocabulary order. */
 void orc_w2v_unigram_table(uint64_t vocab_size, const uint32_t *cn, uint32_t table_size, int32_t *table);

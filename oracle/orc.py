@@ -281,3 +281,24 @@ def eval_subset(deg, v, rnd, seed):
     lib().orc_eval_subset.restype = C.c_uint32
     n = lib().orc_eval_subset(C.c_uint32(deg), C.c_uint32(v), C.c_uint32(rnd), C.c_uint64(seed), P(out))
     return out[:n].copy()
+
+
+W2V_SKIPGRAM, W2V_REUSE, W2V_CBOW = 0, 1, 2
+
+
+def w2v_train_mode(syn0, syn1, offsets, tokens, deg, order, rank, in_vocab, vocab_size, train_words, *, mode,
+                   window, negative, sample, alpha, use_lut=True, compat_break=False, table_size=10_000_000,
+                   threads=0, seed=0, rnd=0):
+    """word2vec restatement with the trainer chosen by mode (W2V_*)."""
+    V, d = syn0.shape
+    pairs = C.c_uint64()
+    lib().orc_w2v_train_mode.restype = C.c_uint64
+    words = lib().orc_w2v_train_mode(
+        C.c_uint32(V), C.c_uint32(d), P(syn0), P(syn1), P(np.ascontiguousarray(offsets, np.uint64)),
+        P(np.ascontiguousarray(tokens, np.uint32)), C.c_uint64(len(offsets) - 1),
+        P(np.ascontiguousarray(deg, np.uint32)), P(np.ascontiguousarray(order, np.uint32)),
+        P(np.ascontiguousarray(rank, np.uint32)), P(np.ascontiguousarray(in_vocab, np.uint8)),
+        C.c_uint64(vocab_size), C.c_uint64(train_words), C.c_uint32(window), C.c_uint32(negative),
+        C.c_float(sample), C.c_float(alpha), C.c_int(int(use_lut)), C.c_int(int(compat_break)),
+        C.c_uint32(table_size), C.c_int(threads), C.byref(pairs), C.c_int(mode), C.c_uint64(seed), C.c_uint32(rnd))
+    return int(words), int(pairs.value)

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     lib = felog.load_library()
     names = felog.header_symbols()
-    assert len(names) == 39
+    assert len(names) == 40
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     out = subprocess.run(["nm", "-D", "--defined-only", felog.LIB_PATH], capture_output=True, text=True).stdout
@@ -42,7 +42,7 @@ def test_default_options_are_the_reference_defaults():
 
 
 def test_options_struct_layout_matches_header():
-    assert ctypes.sizeof(felog.Options) == 104
+    assert ctypes.sizeof(felog.Options) == 112
     assert felog.Options.seed.offset == 72 and felog.Options.device.offset == 92
     assert felog.Options.update_mode.offset == 96 and felog.Options.compat_walk.offset == 100
     assert ctypes.sizeof(felog.TileShape) == 16

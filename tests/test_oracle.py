@@ -735,3 +735,32 @@ def test_w2v_train_with_negatives_and_table_matches_fp64_restatement():
     r0, r1 = _w2v_fp64(syn0, syn1, sent, order, 2, 3, 0.05, table, vs)
     assert np.abs(s0 - r0).max() < 1e-4 and np.abs(s1 - r1).max() < 1e-4
     assert np.abs(r1 - syn1).max() > 1e-3  # the negatives did train
+
+
+@pytest.mark.parametrize("mode", ["skip-gram", "reuse", "cbow"])
+def test_w2v_trainer_modes_match_fp64_restatement(mode):
+    """orc_w2v_train_mode (one thread, no subsampling, sigmoid table) against
+    the FP64 restatement of the same trainer with the same draws."""
+    import w2v_fp64
+    rng = np.random.default_rng(31)
+    V, d = 14, 8
+    deg = np.array([9, 8, 8, 7, 6, 6, 5, 5, 4, 3, 3, 2, 2, 2], np.uint32)
+    order, rank, inv, vs, tw = orc.vocab(deg, 1)
+    syn0 = (rng.standard_normal((V, d)) * 0.4).astype(np.float32)
+    syn1 = (rng.standard_normal((V, d)) * 0.4).astype(np.float32)
+    sent = [rng.integers(0, V, 9).tolist(), rng.integers(0, V, 14).tolist(), rng.integers(0, V, 6).tolist()]
+    offsets = np.cumsum([0] + [len(s) for s in sent]).astype(np.uint64)
+    tokens = np.array(sum(sent, []), np.uint32)
+    T = 1000
+    table = orc.w2v_unigram_table(deg[order], T)
+    m = {"skip-gram": orc.W2V_SKIPGRAM, "reuse": orc.W2V_REUSE, "cbow": orc.W2V_CBOW}[mode]
+    s0, s1 = syn0.copy(), syn1.copy()
+    seed, rnd = 0x1234567890AB, 3
+    words, pairs = orc.w2v_train_mode(s0, s1, offsets, tokens, deg, order, rank, inv, vs, tw, mode=m, window=2,
+                                      negative=3, sample=0.0, alpha=0.05, table_size=T, threads=1, seed=seed,
+                                      rnd=rnd)
+    assert words == len(tokens) and pairs > 0
+    negs = w2v_fp64.reuse_negatives(len(sent), 3, table, vs, order, seed, rnd) if mode == "reuse" else None
+    r0, r1 = w2v_fp64.train(mode, syn0, syn1, sent, range(len(sent)), order, 2, 3, 0.05, table, vs, negs)
+    assert np.abs(s0 - r0).max() < 1e-4 and np.abs(s1 - r1).max() < 1e-4
+    assert np.abs(r1 - syn1).max() > 1e-3 and np.abs(r0 - syn0).max() > 1e-3
