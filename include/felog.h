@@ -277,6 +277,15 @@ felog_status felog_embeddings_copy_in(felog_ctx *ctx, const float *syn0,
 felog_status felog_write_walks(felog_ctx *ctx, const char *prefix);
 felog_status felog_write_embeddings(felog_ctx *ctx, const char *path,
                                     int binary);
+/* -classes (src/word2vec.cu:2128-2175): k-means of the vocabulary's syn0
+ * rows into `classes` classes over `iters` rounds (the reference runs 10),
+ * FP32 tables.  felog_kmeans_classes writes one class per vocabulary entry
+ * (vocabulary order, vocab_size entries); felog_write_classes writes the
+ * reference's "<vertex id> <class>\n" lines instead of the embeddings. */
+felog_status felog_kmeans_classes(felog_ctx *ctx, uint32_t classes, uint32_t iters,
+                                  int32_t *class_of);
+felog_status felog_write_classes(felog_ctx *ctx, const char *path,
+                                 uint32_t classes);
 
 felog_status felog_stats_get(felog_ctx *ctx, felog_stats *out);
 

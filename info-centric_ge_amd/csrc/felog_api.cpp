@@ -1131,6 +1131,37 @@ felog_status felog_write_embeddings(felog_ctx *c, const char *path, int binary) 
     return FELOG_OK;
 }
 
+felog_status felog_kmeans_classes(felog_ctx *c, uint32_t classes, uint32_t iters, int32_t *class_of) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!class_of || classes == 0) return fail(c, FELOG_ERR_ARG, "classes must be > 0 and class_of non-null");
+    if (c->opt.precision != FELOG_FP32) return fail(c, FELOG_ERR_STATE, "k-means runs on FP32 tables");
+    int32_t *d_cl = nullptr;
+    TRY(c, hipMalloc(&d_cl, (c->vocab_size ? c->vocab_size : 1) * 4));
+    hipError_t e = kmeans_classes(c, classes, iters, d_cl);
+    if (e == hipSuccess) e = hipMemcpy(class_of, d_cl, c->vocab_size * 4, hipMemcpyDeviceToHost);
+    dfree(d_cl);
+    TRY(c, e);
+    return FELOG_OK;
+}
+
+// The classes file of src/word2vec.cu:2172: "<vertex id> <class>\n" per
+// vocabulary entry, in vocabulary order, after 10 k-means rounds.
+felog_status felog_write_classes(felog_ctx *c, const char *path, uint32_t classes) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!path) return fail(c, FELOG_ERR_ARG, "null path");
+    std::vector<int32_t> cl(c->vocab_size);
+    if ((s = felog_kmeans_classes(c, classes, 10, cl.data()))) return s;
+    std::vector<uint32_t> order(c->V);
+    TRY(c, hipMemcpy(order.data(), c->d_vocab_order, (uint64_t)c->V * 4, hipMemcpyDeviceToHost));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(c, FELOG_ERR_IO, std::string("cannot open ") + path);
+    for (uint64_t a = 0; a < c->vocab_size; ++a) std::fprintf(f, "%u %d\n", order[a], cl[a]);
+    if (std::fclose(f) != 0) return fail(c, FELOG_ERR_IO, std::string("write failed: ") + path);
+    return FELOG_OK;
+}
+
 felog_status felog_stats_get(felog_ctx *c, felog_stats *out) {
     if (!c || !out) return FELOG_ERR_ARG;
     *out = c->st;

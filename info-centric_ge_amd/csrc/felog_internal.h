@@ -252,6 +252,9 @@ hipError_t w2v_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok);
 hipError_t w2v_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok);
 hipError_t run_w2v(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
                    float *ms);
+// k-means classes of the vocabulary's syn0 rows (FP32 tables) into d_cl
+// (vocabulary order)
+hipError_t kmeans_classes(felog_ctx *c, uint32_t K, uint32_t iters, int32_t *d_cl);
 // eval_kernel.hip
 // gated: evaluate only when the round state says so (captured rounds)
 hipError_t eval_enqueue(felog_ctx *c, bool gated);

@@ -495,3 +495,122 @@ hipError_t run_w2v(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, u
 }
 
 }  // namespace felog
+
+// ---------------------------------------------------------------------------
+// -classes: k-means over the vocabulary's syn0 rows (src/word2vec.cu:2128-2175):
+// row a (vocabulary order) starts in class a % K; each of `iters` rounds
+// sums the rows of every class (FP64 here), divides by (members + 1) as the
+// reference's centcn, which starts at 1, does, scales every centroid to unit
+// length, and moves each row to the class of the largest dot product (the
+// first on ties, starting from -10).
+// ---------------------------------------------------------------------------
+namespace felog {
+namespace {
+
+__global__ void k_km_init(int32_t *__restrict__ cl, uint64_t vs, uint32_t K) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t a = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; a < vs; a += stride) cl[a] = (int32_t)(a % K);
+}
+
+__global__ void k_km_clear(double *__restrict__ sum, unsigned long long *__restrict__ cnt, uint64_t n, uint32_t K) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        sum[i] = 0.0;
+        if (i < K) cnt[i] = 0;
+    }
+}
+
+// One wavefront per row: its columns into the class sums.
+__global__ __launch_bounds__(256) void k_km_accum(const float *__restrict__ syn0, uint32_t dp, uint32_t d,
+                                                  const uint32_t *__restrict__ order, uint64_t vs,
+                                                  const int32_t *__restrict__ cl, double *__restrict__ sum,
+                                                  unsigned long long *__restrict__ cnt) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t a = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; a < vs; a += nwaves) {
+        const float *row = syn0 + (uint64_t)order[a] * dp;
+        const int32_t c = cl[a];
+        for (uint32_t e = lane; e < d; e += 64) atomicAdd(&sum[(uint64_t)c * d + e], (double)row[e]);
+        if (lane == 0) atomicAdd(&cnt[c], 1ull);
+    }
+}
+
+// One wavefront per class: mean (members + 1 in the divisor), unit length.
+__global__ __launch_bounds__(64) void k_km_norm(const double *__restrict__ sum, const unsigned long long *__restrict__ cnt,
+                                                uint32_t K, uint32_t d, float *__restrict__ cent) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t k = blockIdx.x; k < K; k += gridDim.x) {
+        const double div = (double)(cnt[k] + 1);
+        double ss = 0.0;
+        for (uint32_t e = lane; e < d; e += 64) {
+            const double x = sum[(uint64_t)k * d + e] / div;
+            ss += x * x;
+        }
+        for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+        // an empty class divides 0 by 0, as the reference does: its NaN
+        // centroid never wins an assignment
+        const double nrm = sqrt(ss);
+        for (uint32_t e = lane; e < d; e += 64) {
+            const double x = sum[(uint64_t)k * d + e] / div;
+            cent[(uint64_t)k * d + e] = (float)(x / nrm);
+        }
+    }
+}
+
+// One wavefront per row: the class of the largest dot product.
+__global__ __launch_bounds__(256) void k_km_assign(const float *__restrict__ syn0, uint32_t dp, uint32_t d,
+                                                   const uint32_t *__restrict__ order, uint64_t vs,
+                                                   const float *__restrict__ cent, uint32_t K,
+                                                   int32_t *__restrict__ cl) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t a = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; a < vs; a += nwaves) {
+        const float *row = syn0 + (uint64_t)order[a] * dp;
+        float x[8];
+        const uint32_t ne = (d + 63) / 64;
+        for (uint32_t t = 0; t < 8; ++t) x[t] = (t < ne && lane + 64 * t < d) ? row[lane + 64 * t] : 0.f;
+        float best = -10.f;
+        int32_t bid = 0;
+        for (uint32_t k = 0; k < K; ++k) {
+            const float *cr = cent + (uint64_t)k * d;
+            float p = 0.f;
+            for (uint32_t t = 0; t < ne; ++t)
+                if (lane + 64 * t < d) p += x[t] * cr[lane + 64 * t];
+            for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o);
+            if (p > best) {
+                best = p;
+                bid = (int32_t)k;
+            }
+        }
+        if (lane == 0) cl[a] = bid;
+    }
+}
+
+}  // namespace
+
+hipError_t kmeans_classes(felog_ctx *c, uint32_t K, uint32_t iters, int32_t *d_cl) {
+    const uint64_t vs = c->vocab_size;
+    const uint32_t d = c->d;
+    double *sum = nullptr;
+    unsigned long long *cnt = nullptr;
+    float *cent = nullptr;
+    HIP_TRY(hipMalloc(&sum, (uint64_t)K * d * 8));
+    HIP_TRY(hipMalloc(&cnt, (uint64_t)K * 8));
+    HIP_TRY(hipMalloc(&cent, (uint64_t)K * d * 4));
+    const unsigned g = grid_for(vs * 64, 256, 1u << 16);
+    k_km_init<<<grid_for(vs, 256, 65535), 256, 0, c->stream>>>(d_cl, vs, K);
+    for (uint32_t it = 0; it < iters; ++it) {
+        k_km_clear<<<grid_for((uint64_t)K * d, 256, 65535), 256, 0, c->stream>>>(sum, cnt, (uint64_t)K * d, K);
+        k_km_accum<<<g, 256, 0, c->stream>>>((const float *)c->d_syn0, c->dp, d, c->d_vocab_order, vs, d_cl, sum, cnt);
+        k_km_norm<<<K < 65535 ? K : 65535, 64, 0, c->stream>>>(sum, cnt, K, d, cent);
+        k_km_assign<<<g, 256, 0, c->stream>>>((const float *)c->d_syn0, c->dp, d, c->d_vocab_order, vs, cent, K, d_cl);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    dfree(sum);
+    dfree(cnt);
+    dfree(cent);
+    return hipSuccess;
+}
+
+}  // namespace felog

@@ -125,6 +125,8 @@ def load_library(path=LIB_PATH):
         "felog_unigram_copy_out": (C.c_int32, [P, P, P]),
         "felog_flags_copy_out": (C.c_int32, [P, P, P]),
         "felog_w2v_table_copy_out": (C.c_int32, [P, P]),
+        "felog_kmeans_classes": (C.c_int32, [P, C.c_uint32, C.c_uint32, P]),
+        "felog_write_classes": (C.c_int32, [P, C.c_char_p, C.c_uint32]),
         "felog_embeddings_copy_out": (C.c_int32, [P, P, P]),
         "felog_embeddings_copy_in": (C.c_int32, [P, P, P]),
         "felog_write_walks": (C.c_int32, [P, C.c_char_p]),
@@ -386,6 +388,16 @@ class Felog:
         s = np.empty(self.V, np.float32)
         self._check(self.lib.felog_flags_copy_out(self.ctx, _ptr(f), _ptr(s)), "felog_flags_copy_out")
         return f, s
+
+    def kmeans_classes(self, classes, iters=10):
+        """k-means class of every vocabulary entry (vocabulary order)."""
+        vs = self.stats()["vocab_size"]
+        out = np.empty(max(vs, 1), np.int32)
+        self._check(self.lib.felog_kmeans_classes(self.ctx, classes, iters, _ptr(out)), "felog_kmeans_classes")
+        return out[:vs]
+
+    def write_classes(self, path, classes):
+        self._check(self.lib.felog_write_classes(self.ctx, path.encode(), classes), "felog_write_classes")
 
     def w2v_table(self):
         """The per-pair trainers' unigram table (vocabulary indices)."""

@@ -145,8 +145,16 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "felog: -emb_output <file> is required\n");
         return 1;
     }
-    if (hs || classes) {
-        std::fprintf(stderr, "felog: -hs and -classes are not part of this build\n");
+    if (hs) {
+        std::fprintf(stderr, "felog: -hs (hierarchical softmax) is not part of this build\n");
+        return 1;
+    }
+    if (classes < 0) {
+        std::fprintf(stderr, "felog: -classes must be >= 0\n");
+        return 1;
+    }
+    if (classes && o.precision != FELOG_FP32) {
+        std::fprintf(stderr, "felog: -classes runs k-means on FP32 tables\n");
         return 1;
     }
     int reuse_neg = 1;  // src/word2vec.cu:163
@@ -201,7 +209,12 @@ int main(int argc, char **argv) {
         }
         if (corpus.n_walks == 0) break;
     }
-    if ((s = felog_write_embeddings(c, emb.c_str(), o.binary))) return die(c, "write embeddings", s);
+    // src/word2vec.cu:2118-2172: the vectors, or with -classes their k-means classes
+    if (classes) {
+        if ((s = felog_write_classes(c, emb.c_str(), (uint32_t)classes))) return die(c, "write classes", s);
+    } else if ((s = felog_write_embeddings(c, emb.c_str(), o.binary))) {
+        return die(c, "write embeddings", s);
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     felog_stats_get(c, &st);
     std::printf("walk time %.3f s, train time %.3f s, whole time %.3f s\n", st.walk_ms_total / 1e3,

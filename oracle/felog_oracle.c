@@ -959,6 +959,50 @@ uint32_t orc_eval(uint32_t V, uint32_t d, const float *syn0, const uint64_t *row
     return eva;
 }
 
+/* ---- -classes: k-means, src/word2vec.cu:2128-2175 ----------------------
+ * Rows in vocabulary order (row a = syn0 row of vertex order[a]); float
+ * arithmetic in the reference's order. */
+void orc_kmeans_classes(uint32_t d, const float *syn0, const uint32_t *order, uint64_t vocab_size, uint32_t classes,
+                        uint32_t iter, int32_t *cl) {
+    int *centcn = (int *)malloc(classes * sizeof(int));
+    float *cent = (float *)calloc((uint64_t)classes * d, sizeof(float));
+    for (uint64_t a = 0; a < vocab_size; ++a) cl[a] = (int32_t)(a % classes);
+    for (uint32_t it = 0; it < iter; ++it) {
+        for (uint64_t b = 0; b < (uint64_t)classes * d; ++b) cent[b] = 0;
+        for (uint32_t b = 0; b < classes; ++b) centcn[b] = 1;
+        for (uint64_t c = 0; c < vocab_size; ++c) {
+            const float *x = syn0 + (uint64_t)order[c] * d;
+            for (uint32_t e = 0; e < d; ++e) cent[(uint64_t)d * cl[c] + e] += x[e];
+            centcn[cl[c]]++;
+        }
+        for (uint32_t b = 0; b < classes; ++b) {
+            float closev = 0;
+            for (uint32_t c = 0; c < d; ++c) {
+                cent[(uint64_t)d * b + c] /= centcn[b];
+                closev += cent[(uint64_t)d * b + c] * cent[(uint64_t)d * b + c];
+            }
+            closev = sqrtf(closev);
+            for (uint32_t c = 0; c < d; ++c) cent[(uint64_t)d * b + c] /= closev;
+        }
+        for (uint64_t c = 0; c < vocab_size; ++c) {
+            const float *x = syn0 + (uint64_t)order[c] * d;
+            float closev = -10;
+            int32_t closeid = 0;
+            for (uint32_t k = 0; k < classes; ++k) {
+                float v = 0;
+                for (uint32_t e = 0; e < d; ++e) v += cent[(uint64_t)d * k + e] * x[e];
+                if (v > closev) {
+                    closev = v;
+                    closeid = (int32_t)k;
+                }
+            }
+            cl[c] = closeid;
+        }
+    }
+    free(centcn);
+    free(cent);
+}
+
 /* ---- AUC -------------------------------------------------------------- */
 typedef struct { float s; int pos; } scored_t;
 static int by_score(const void *a, const void *b) {

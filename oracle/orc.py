@@ -283,6 +283,16 @@ def eval_subset(deg, v, rnd, seed):
     return out[:n].copy()
 
 
+def kmeans_classes(syn0, order, vocab_size, classes, iters=10):
+    """-classes k-means restated; classes of the vocabulary in vocabulary order."""
+    V, d = syn0.shape
+    cl = np.empty(max(vocab_size, 1), np.int32)
+    lib().orc_kmeans_classes(C.c_uint32(d), P(np.ascontiguousarray(syn0, np.float32)),
+                             P(np.ascontiguousarray(order, np.uint32)), C.c_uint64(vocab_size), C.c_uint32(classes),
+                             C.c_uint32(iters), P(cl))
+    return cl[:vocab_size]
+
+
 W2V_SKIPGRAM, W2V_REUSE, W2V_CBOW = 0, 1, 2
 
 

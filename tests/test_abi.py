@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol():
     lib = felog.load_library()
     names = felog.header_symbols()
-    assert len(names) == 40
+    assert len(names) == 42
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     out = subprocess.run(["nm", "-D", "--defined-only", felog.LIB_PATH], capture_output=True, text=True).stdout

@@ -219,6 +219,27 @@ def test_cli_end_to_end(tmp_path):
     head = open(emb).readline().split()
     assert int(head[1]) == 32 and int(head[0]) > 900
     assert os.path.exists(str(tmp_path / "w") + "-0-1.txt")
+    assert "HogBatch" in r.stdout
+
+
+@pytest.mark.parametrize("flags,trainer", [([], "CBOW"), (["-cbow", "0"], "per-sentence negatives"),
+                                           (["-cbow", "0", "-reuse-neg", "0", "--felog-compat-sgns"], "per pair")])
+def test_cli_reference_defaults_pick_the_reference_trainers(tmp_path, flags, trainer):
+    """The reference's -cbow / -reuse-neg defaults (1, 1) select CBOW, -cbow 0
+    the negative-reuse rule; -classes writes k-means classes."""
+    src, dst, w = graphgen.erdos_renyi(1000, 5000, seed=9)
+    g = str(tmp_path / "g.bin")
+    graphgen.write_bin(g, src, dst, w)
+    out = str(tmp_path / "classes.txt")
+    exe = os.path.join(ROOT, "info-centric_ge_amd", "felog")
+    cmd = [exe, "-g", g, "-v", "1000", "-w", "1000", "--min_L", "20", "--min_R", "2", "--make-undirected",
+           "-o", str(tmp_path / "w"), "-size", "32", "-window", "5", "-negative", "5", "-emb_output", out,
+           "--max-rounds", "2", "-classes", "7"] + flags
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and trainer in r.stdout
+    rows = [l.split() for l in open(out).read().splitlines()]
+    assert len(rows) > 900 and all(len(x) == 2 and 0 <= int(x[1]) < 7 for x in rows)
 
 
 def _auc(emb, pos, neg):

@@ -87,6 +87,35 @@ def test_single_sentences_match_fp64(mode):
         f.close()
 
 
+def test_kmeans_classes_match_oracle(tmp_path):
+    """-classes on well-separated synthetic rows: the device k-means gives the
+    oracle's classes (src/word2vec.cu:2128-2175) after 1 and 10 rounds, and
+    the classes file holds "<id> <class>" per vocabulary entry in order."""
+    n = 5000
+    src, dst, w = graphgen.erdos_renyi(n, 25_000, seed=43)
+    f = make(src, dst, w, n)
+    try:
+        rng = np.random.default_rng(5)
+        centres = rng.standard_normal((8, 64)) * 4
+        rows = (centres[rng.integers(0, 8, n)] + 0.05 * rng.standard_normal((n, 64))).astype(np.float32)
+        f.set_embeddings(rows, None)
+        order, deg = f.vocab()
+        _, _, inv, vs, _ = orc.vocab(deg, 5)
+        for it in (1, 10):
+            got = f.kmeans_classes(8, it)
+            ref = orc.kmeans_classes(rows, order, vs, 8, it)
+            assert np.array_equal(got, ref), it
+        path = str(tmp_path / "classes.txt")
+        f.write_classes(path, 8)
+        lines = open(path).read().splitlines()
+        assert len(lines) == vs
+        ids = np.array([int(l.split()[0]) for l in lines])
+        cls = np.array([int(l.split()[1]) for l in lines])
+        assert np.array_equal(ids, order[:vs]) and np.array_equal(cls, ref)
+    finally:
+        f.close()
+
+
 @pytest.fixture(scope="module")
 def lp_rmat_1m():
     n = 1_100_000

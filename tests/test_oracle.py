@@ -764,3 +764,36 @@ def test_w2v_trainer_modes_match_fp64_restatement(mode):
     r0, r1 = w2v_fp64.train(mode, syn0, syn1, sent, range(len(sent)), order, 2, 3, 0.05, table, vs, negs)
     assert np.abs(s0 - r0).max() < 1e-4 and np.abs(s1 - r1).max() < 1e-4
     assert np.abs(r1 - syn1).max() > 1e-3 and np.abs(r0 - syn0).max() > 1e-3
+
+
+# ---- -classes k-means restatement
+def test_kmeans_by_hand_and_separated_clusters():
+    """One round by hand on four rows; on well-separated clusters each round
+    moves every row to the best unit centroid of the previous round's classes
+    and no true cluster is split."""
+    syn0 = np.array([[1, 0], [0.9, 0.1], [0, 1], [0.2, 0.8]], np.float32)
+    order = np.arange(4, dtype=np.uint32)
+    # start: classes 0,1,0,1; centroid 0 = (row0+row2)/3, centroid 1 = (row1+row3)/3, unit length
+    c0 = (syn0[0] + syn0[2]) / 3
+    c1 = (syn0[1] + syn0[3]) / 3
+    c0, c1 = c0 / np.linalg.norm(c0), c1 / np.linalg.norm(c1)
+    want = [int(np.argmax([x @ c0, x @ c1])) for x in syn0]
+    assert orc.kmeans_classes(syn0, order, 4, 2, iters=1).tolist() == want
+    rng = np.random.default_rng(2)
+    centres = rng.standard_normal((6, 16)) * 5
+    labels = rng.integers(0, 6, 3000)
+    x = (centres[labels] + 0.05 * rng.standard_normal((3000, 16))).astype(np.float32)
+    perm = rng.permutation(3000).astype(np.uint32)
+    # round 10 moves each row to the best unit centroid of round 9's classes
+    c9 = orc.kmeans_classes(x, perm, 3000, 6, iters=9)
+    c10 = orc.kmeans_classes(x, perm, 3000, 6, iters=10)
+    rows = x[perm].astype(np.float64)
+    cent = np.stack([rows[c9 == k].sum(0) / ((c9 == k).sum() + 1) for k in range(6)])
+    with np.errstate(invalid="ignore"):
+        cent /= np.linalg.norm(cent, axis=1, keepdims=True)  # an empty class: 0/0, never chosen
+    dots = np.nan_to_num(rows @ cent.T, nan=-np.inf)
+    best = np.where(dots.max(1) > -10, dots.argmax(1), 0)
+    assert np.array_equal(c10, best)
+    # rows of one true cluster never split
+    for lab in range(6):
+        assert len(np.unique(c10[labels[perm] == lab])) == 1
