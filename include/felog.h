@@ -277,6 +277,44 @@ felog_status felog_embeddings_copy_in(felog_ctx *ctx, const float *syn0,
 felog_status felog_write_walks(felog_ctx *ctx, const char *prefix);
 felog_status felog_write_embeddings(felog_ctx *ctx, const char *path,
                                     int binary);
+/* ---- the steps either side of the path (src/tools/, include/storage.hpp) */
+/* Text edge lists (include/storage.hpp:156-164, src/tools/gconverter.cpp:
+ * 57-76): whitespace-separated "src dst" (columns 2) or "src dst data"
+ * (columns 3) records, read until the first record that does not parse.
+ * felog_read_edges_txt needs no context: with src NULL it only counts (*m);
+ * otherwise it fills up to `capacity` records (w may be NULL).
+ * felog_load_edges_txt = felog_load_edges for such a file (weighted: 3
+ * columns; else every weight is 1). */
+felog_status felog_read_edges_txt(const char *path, int columns, uint32_t *src,
+                                  uint32_t *dst, float *w, uint64_t capacity,
+                                  uint64_t *m);
+felog_status felog_load_edges_txt(felog_ctx *ctx, const char *path, uint32_t v_num,
+                                  int make_undirected, int weighted);
+/* Writers of include/storage.hpp:207-240 ("%u %u %f\n", or "%u %u\n" with w
+ * NULL) and :107-115 (binary records of 12 B, or 8 B with w NULL). */
+felog_status felog_write_edges_txt(const char *path, const uint32_t *src,
+                                   const uint32_t *dst, const float *w, uint64_t m);
+felog_status felog_write_edges_bin(const char *path, const uint32_t *src,
+                                   const uint32_t *dst, const float *w, uint64_t m);
+/* Common neighbours (src/tools/gcn.cpp:304-357) of m vertex pairs over the
+ * built graph: the distinct vertices adjacent to both, the pair's own two
+ * vertices excluded (the reference's count on graphs without self loops). */
+felog_status felog_common_neighbours(felog_ctx *ctx, const uint32_t *src,
+                                     const uint32_t *dst, uint64_t m, int32_t *cn);
+/* on != 0: count the common neighbours of every CSR entry's two endpoints on
+ * the device and let the reference's transition (compat_walk) read them in
+ * place of the edge data, as the reference reads gcn's output; 0: back to
+ * the edge data.  felog_cn_copy_out returns the counts in CSR order. */
+felog_status felog_use_common_neighbours(felog_ctx *ctx, int on);
+felog_status felog_cn_copy_out(felog_ctx *ctx, float *cn);
+/* Held-out link prediction (the test files of src/tools/gtrain.cpp:369-384):
+ * pairs holds n vertex pairs (2n ids), labels 1 / 0; *auc = the
+ * Mann-Whitney AUC (ties at half weight) of the cosine of the pairs' syn0
+ * rows; scores (may be NULL) receives the cosines. */
+felog_status felog_link_auc(felog_ctx *ctx, const uint32_t *pairs,
+                            const uint8_t *labels, uint64_t n, float *scores,
+                            double *auc);
+
 /* -classes (src/word2vec.cu:2128-2175): k-means of the vocabulary's syn0
  * rows into `classes` classes over `iters` rounds (the reference runs 10),
  * FP32 tables.  felog_kmeans_classes writes one class per vocabulary entry

@@ -15,6 +15,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cctype>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -72,7 +74,8 @@ void free_all(felog_ctx *c) {
                     c->d_scores, c->d_active, c->d_sel, c->d_slots, c->d_lens, c->d_ovf_base, c->d_ovf,
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
                     c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats,
-                    c->d_w2v_ftok, c->d_w2v_kcnt, c->d_w2v_blk, c->d_w2v_table, c->d_w2v_exp};
+                    c->d_w2v_ftok, c->d_w2v_kcnt, c->d_w2v_blk, c->d_w2v_table, c->d_w2v_exp, c->d_scol,
+                    c->d_cn};
     for (void *p : ptrs) dfree(p);
     drop_round_graph(c);
     dfree(c->d_rs);
@@ -1128,6 +1131,193 @@ felog_status felog_write_embeddings(felog_ctx *c, const char *path, int binary) 
         std::fputc('\n', f);
     }
     if (std::fclose(f) != 0) return fail(c, FELOG_ERR_IO, std::string("write failed: ") + path);
+    return FELOG_OK;
+}
+
+// ---- the tools either side of the path --------------------------------------
+
+}  // extern "C"
+
+namespace {
+// Whitespace-separated edge records "src dst" or "src dst data" read as
+// fscanf("%u %u") / fscanf("%u %u %f") would (include/storage.hpp:156-164):
+// records are taken until the first one that does not parse.  With src
+// null only the records are counted.
+felog_status parse_edges_txt(felog_ctx *c, const char *path, int columns, uint32_t *src, uint32_t *dst, float *w,
+                             uint64_t cap, uint64_t *m) {
+    if (!path || !m || (columns != 2 && columns != 3)) return fail(c, FELOG_ERR_ARG, "bad arguments");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(c, FELOG_ERR_IO, std::string("cannot open ") + path);
+    std::string buf;
+    char chunk[1 << 16];
+    size_t got;
+    while ((got = std::fread(chunk, 1, sizeof chunk, f)) > 0) buf.append(chunk, got);
+    std::fclose(f);
+    const char *p = buf.c_str();
+    uint64_t n = 0;
+    while (true) {
+        char *end = nullptr;
+        uint32_t rec[2];
+        bool ok = true;
+        for (int k = 0; k < 2 && ok; ++k) {
+            while (*p && std::isspace((unsigned char)*p)) ++p;
+            if (!*p || *p == '-' || !std::isdigit((unsigned char)*p)) { ok = false; break; }
+            const unsigned long x = std::strtoul(p, &end, 10);
+            if (end == p || x > 0xFFFFFFFFul) { ok = false; break; }
+            rec[k] = (uint32_t)x;
+            p = end;
+        }
+        float data = 1.f;
+        if (ok && columns == 3) {
+            const float x = std::strtof(p, &end);
+            if (end == p) ok = false;
+            else {
+                data = x;
+                p = end;
+            }
+        }
+        if (!ok) break;
+        if (src) {
+            if (n >= cap) return fail(c, FELOG_ERR_CAPACITY, "more edges than the buffers hold");
+            src[n] = rec[0];
+            dst[n] = rec[1];
+            if (w) w[n] = data;
+        }
+        ++n;
+    }
+    *m = n;
+    return FELOG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+felog_status felog_read_edges_txt(const char *path, int columns, uint32_t *src, uint32_t *dst, float *w,
+                                  uint64_t capacity, uint64_t *m) {
+    felog_ctx scratch_err;  // parse errors have no context to report to
+    return parse_edges_txt(&scratch_err, path, columns, src, dst, w, capacity, m);
+}
+
+felog_status felog_load_edges_txt(felog_ctx *c, const char *path, uint32_t v_num, int make_undirected, int weighted) {
+    if (!c) return FELOG_ERR_ARG;
+    const int cols = weighted ? 3 : 2;
+    uint64_t m = 0;
+    felog_status s = parse_edges_txt(c, path, cols, nullptr, nullptr, nullptr, 0, &m);
+    if (s) return s;
+    std::vector<uint32_t> src(m), dst(m);
+    std::vector<float> w(m, 1.f);
+    if ((s = parse_edges_txt(c, path, cols, src.data(), dst.data(), weighted ? w.data() : nullptr, m, &m))) return s;
+    return felog_set_edges(c, src.data(), dst.data(), w.data(), m, v_num, make_undirected);
+}
+
+// include/storage.hpp:207-240: "%u %u\n" (no data) or "%u %u %f\n" per edge.
+felog_status felog_write_edges_txt(const char *path, const uint32_t *src, const uint32_t *dst, const float *w,
+                                   uint64_t m) {
+    if (!path || (m && (!src || !dst))) return FELOG_ERR_ARG;
+    FILE *f = std::fopen(path, "w");
+    if (!f) return FELOG_ERR_IO;
+    for (uint64_t i = 0; i < m; ++i) {
+        if (w) std::fprintf(f, "%u %u %f\n", src[i], dst[i], (double)w[i]);
+        else std::fprintf(f, "%u %u\n", src[i], dst[i]);
+    }
+    return std::fclose(f) == 0 ? FELOG_OK : FELOG_ERR_IO;
+}
+
+// include/storage.hpp:107-115 (write_graph): packed little-endian records,
+// {u32 src, u32 dst, f32 data} (12 B) or without data {u32, u32} (8 B).
+felog_status felog_write_edges_bin(const char *path, const uint32_t *src, const uint32_t *dst, const float *w,
+                                   uint64_t m) {
+    if (!path || (m && (!src || !dst))) return FELOG_ERR_ARG;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return FELOG_ERR_IO;
+    std::vector<unsigned char> rec;
+    const size_t rs = w ? 12 : 8;
+    const uint64_t step = 1 << 20;
+    for (uint64_t i0 = 0; i0 < m; i0 += step) {
+        const uint64_t k = m - i0 < step ? m - i0 : step;
+        rec.resize(k * rs);
+        for (uint64_t i = 0; i < k; ++i) {
+            std::memcpy(&rec[i * rs], &src[i0 + i], 4);
+            std::memcpy(&rec[i * rs + 4], &dst[i0 + i], 4);
+            if (w) std::memcpy(&rec[i * rs + 8], &w[i0 + i], 4);
+        }
+        if (std::fwrite(rec.data(), 1, rec.size(), f) != rec.size()) {
+            std::fclose(f);
+            return FELOG_ERR_IO;
+        }
+    }
+    return std::fclose(f) == 0 ? FELOG_OK : FELOG_ERR_IO;
+}
+
+felog_status felog_common_neighbours(felog_ctx *c, const uint32_t *src, const uint32_t *dst, uint64_t m,
+                                     int32_t *cn) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (m == 0) return FELOG_OK;
+    if (!src || !dst || !cn) return fail(c, FELOG_ERR_ARG, "null pair or output array");
+    for (uint64_t i = 0; i < m; ++i)
+        if (src[i] >= c->V || dst[i] >= c->V) return fail(c, FELOG_ERR_ARG, "vertex id >= v_num");
+    uint32_t *du = nullptr, *dv = nullptr;
+    int32_t *dout = nullptr;
+    TRY(c, hipMalloc(&du, m * 4));
+    TRY(c, hipMalloc(&dv, m * 4));
+    TRY(c, hipMalloc(&dout, m * 4));
+    hipError_t e = hipMemcpy(du, src, m * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dv, dst, m * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = common_neighbours_pairs(c, du, dv, m, dout);
+    if (e == hipSuccess) e = hipMemcpy(cn, dout, m * 4, hipMemcpyDeviceToHost);
+    dfree(du);
+    dfree(dv);
+    dfree(dout);
+    TRY(c, e);
+    return FELOG_OK;
+}
+
+felog_status felog_use_common_neighbours(felog_ctx *c, int on) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    drop_round_graph(c);  // the captured walk holds the edge-data pointer
+    if (!on) {
+        dfree(c->d_cn);
+        c->d_cn = nullptr;
+        return FELOG_OK;
+    }
+    TRY(c, common_neighbours_csr(c));
+    return FELOG_OK;
+}
+
+felog_status felog_cn_copy_out(felog_ctx *c, float *cn) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!cn) return fail(c, FELOG_ERR_ARG, "null output");
+    if (!c->d_cn) return fail(c, FELOG_ERR_STATE, "no common-neighbour counts: call felog_use_common_neighbours");
+    if (c->E) TRY(c, hipMemcpy(cn, c->d_cn, c->E * 4, hipMemcpyDeviceToHost));
+    return FELOG_OK;
+}
+
+felog_status felog_link_auc(felog_ctx *c, const uint32_t *pairs, const uint8_t *labels, uint64_t n, float *scores,
+                            double *auc) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!pairs || !labels || !auc || n == 0) return fail(c, FELOG_ERR_ARG, "pairs, labels and auc are required");
+    for (uint64_t i = 0; i < 2 * n; ++i)
+        if (pairs[i] >= c->V) return fail(c, FELOG_ERR_ARG, "vertex id >= v_num");
+    uint32_t *dp = nullptr;
+    uint8_t *dl = nullptr;
+    float *ds = nullptr;
+    TRY(c, hipMalloc(&dp, n * 8));
+    TRY(c, hipMalloc(&dl, n));
+    TRY(c, hipMalloc(&ds, n * 4));
+    std::vector<uint8_t> lab(labels, labels + n);
+    for (auto &x : lab) x = x ? 1 : 0;
+    hipError_t e = hipMemcpy(dp, pairs, n * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dl, lab.data(), n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = link_auc(c, dp, dl, n, ds, auc);
+    if (e == hipSuccess && scores) e = hipMemcpy(scores, ds, n * 4, hipMemcpyDeviceToHost);
+    dfree(dp);
+    dfree(dl);
+    dfree(ds);
+    TRY(c, e);
     return FELOG_OK;
 }
 

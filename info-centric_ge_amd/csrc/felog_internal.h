@@ -171,6 +171,10 @@ struct felog_ctx {
     int32_t *d_w2v_table = nullptr;
     uint32_t w2v_table_size = 0;
     float *d_w2v_exp = nullptr;
+    // common neighbours: the CSR sorted by neighbour, and per CSR entry the
+    // count the reference's transition reads (null: the edge data)
+    uint32_t *d_scol = nullptr;
+    float *d_cn = nullptr;
 
     felog_stats st{};
 };
@@ -252,6 +256,14 @@ hipError_t w2v_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok);
 hipError_t w2v_enqueue(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok);
 hipError_t run_w2v(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, uint64_t n_sent, uint64_t n_tok,
                    float *ms);
+// graph_tools.hip: common neighbours of n pairs (device arrays), of every
+// CSR entry (into d_cn), and the held-out link-prediction AUC of n pairs
+// (2n vertex ids, labels 0/1) from the cosine of their syn0 rows
+hipError_t common_neighbours_pairs(felog_ctx *c, const uint32_t *d_u, const uint32_t *d_v, uint64_t n,
+                                   int32_t *d_out);
+hipError_t common_neighbours_csr(felog_ctx *c);
+hipError_t link_auc(felog_ctx *c, const uint32_t *d_pairs, const uint8_t *d_lab, uint64_t n, float *d_score,
+                    double *auc);
 // k-means classes of the vocabulary's syn0 rows (FP32 tables) into d_cl
 // (vocabulary order)
 hipError_t kmeans_classes(felog_ctx *c, uint32_t K, uint32_t iters, int32_t *d_cl);

@@ -359,7 +359,9 @@ hipError_t walk_enqueue(felog_ctx *c) {
     a.T = c->d_T;
     a.dT = c->d_dT;
     a.fixed_len = c->budget_on ? c->d_budget_len : nullptr;
-    a.cn_data = c->opt.compat_walk ? c->d_w : nullptr;
+    // the reference's transition reads common-neighbour counts from the edge
+    // data, or from the counts felog_use_common_neighbours computed
+    a.cn_data = c->opt.compat_walk ? (c->d_cn ? c->d_cn : c->d_w) : nullptr;
     a.min_L = c->opt.min_L;
     a.max_len = c->opt.max_walk_len;
     a.seed = c->opt.seed;

@@ -126,6 +126,14 @@ def load_library(path=LIB_PATH):
         "felog_flags_copy_out": (C.c_int32, [P, P, P]),
         "felog_w2v_table_copy_out": (C.c_int32, [P, P]),
         "felog_kmeans_classes": (C.c_int32, [P, C.c_uint32, C.c_uint32, P]),
+        "felog_read_edges_txt": (C.c_int32, [C.c_char_p, C.c_int, P, P, P, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "felog_load_edges_txt": (C.c_int32, [P, C.c_char_p, C.c_uint32, C.c_int, C.c_int]),
+        "felog_write_edges_txt": (C.c_int32, [C.c_char_p, P, P, P, C.c_uint64]),
+        "felog_write_edges_bin": (C.c_int32, [C.c_char_p, P, P, P, C.c_uint64]),
+        "felog_common_neighbours": (C.c_int32, [P, P, P, C.c_uint64, P]),
+        "felog_use_common_neighbours": (C.c_int32, [P, C.c_int]),
+        "felog_cn_copy_out": (C.c_int32, [P, P]),
+        "felog_link_auc": (C.c_int32, [P, P, P, C.c_uint64, P, C.POINTER(C.c_double)]),
         "felog_write_classes": (C.c_int32, [P, C.c_char_p, C.c_uint32]),
         "felog_embeddings_copy_out": (C.c_int32, [P, P, P]),
         "felog_embeddings_copy_in": (C.c_int32, [P, P, P]),
@@ -161,6 +169,36 @@ def load_library(path=LIB_PATH):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def read_edges_txt(path, columns=3):
+    """Text edge list (src dst [data]) through the library's reader; no
+    context needed.  Returns (src, dst, w) with w None for two columns."""
+    lib = load_library()
+    m = C.c_uint64()
+    st = lib.felog_read_edges_txt(path.encode(), columns, None, None, None, 0, C.byref(m))
+    if st != 0:
+        raise FelogError(st, "felog_read_edges_txt", path)
+    src = np.empty(max(m.value, 1), np.uint32)
+    dst = np.empty(max(m.value, 1), np.uint32)
+    w = np.empty(max(m.value, 1), np.float32) if columns == 3 else None
+    st = lib.felog_read_edges_txt(path.encode(), columns, _ptr(src), _ptr(dst), _ptr(w), m.value, C.byref(m))
+    if st != 0:
+        raise FelogError(st, "felog_read_edges_txt", path)
+    n = m.value
+    return src[:n], dst[:n], (w[:n] if w is not None else None)
+
+
+def write_edges(path, src, dst, w=None, binary=True):
+    """Binary graph file (12- or 8-byte records) or text edge list."""
+    lib = load_library()
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    w = None if w is None else np.ascontiguousarray(w, np.float32)
+    fn = lib.felog_write_edges_bin if binary else lib.felog_write_edges_txt
+    st = fn(path.encode(), _ptr(src), _ptr(dst), _ptr(w), len(src))
+    if st != 0:
+        raise FelogError(st, "felog_write_edges", path)
 
 
 def default_options():
@@ -388,6 +426,40 @@ class Felog:
         s = np.empty(self.V, np.float32)
         self._check(self.lib.felog_flags_copy_out(self.ctx, _ptr(f), _ptr(s)), "felog_flags_copy_out")
         return f, s
+
+    def load_edges_txt(self, path, v_num, make_undirected=True, weighted=True):
+        self._check(self.lib.felog_load_edges_txt(self.ctx, path.encode(), v_num, int(make_undirected),
+                                                  int(weighted)), "felog_load_edges_txt")
+        self.V = v_num
+        m = C.c_uint64()
+        self.lib.felog_read_edges_txt(path.encode(), 3 if weighted else 2, None, None, None, 0, C.byref(m))
+        self.E = m.value * (2 if make_undirected else 1)
+
+    def common_neighbours(self, src, dst):
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        out = np.empty(max(len(src), 1), np.int32)
+        self._check(self.lib.felog_common_neighbours(self.ctx, _ptr(src), _ptr(dst), len(src), _ptr(out)),
+                    "felog_common_neighbours")
+        return out[:len(src)]
+
+    def use_common_neighbours(self, on=True):
+        self._check(self.lib.felog_use_common_neighbours(self.ctx, int(on)), "felog_use_common_neighbours")
+
+    def cn(self):
+        out = np.empty(max(self.E, 1), np.float32)
+        self._check(self.lib.felog_cn_copy_out(self.ctx, _ptr(out)), "felog_cn_copy_out")
+        return out[:self.E]
+
+    def link_auc(self, pairs, labels):
+        """(AUC, scores) of n vertex pairs (n x 2) with 0/1 labels."""
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        labels = np.ascontiguousarray(labels, np.uint8)
+        scores = np.empty(len(pairs), np.float32)
+        auc = C.c_double()
+        self._check(self.lib.felog_link_auc(self.ctx, _ptr(pairs), _ptr(labels), len(pairs), _ptr(scores),
+                                            C.byref(auc)), "felog_link_auc")
+        return auc.value, scores
 
     def kmeans_classes(self, classes, iters=10):
         """k-means class of every vocabulary entry (vocabulary order)."""

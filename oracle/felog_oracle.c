@@ -959,6 +959,41 @@ uint32_t orc_eval(uint32_t V, uint32_t d, const float *syn0, const uint64_t *row
     return eva;
 }
 
+/* ---- common neighbours, src/tools/gcn.cpp:304-357 ----------------------
+ * Each row's neighbours sorted (the reference's sort_csr), the two rows
+ * merged; the count skips repeated entries and the pair's own vertices. */
+static int cmp_u32(const void *a, const void *b) {
+    const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+    return (x > y) - (x < y);
+}
+
+void orc_common_neighbours(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, uint64_t m, const uint32_t *u,
+                           const uint32_t *v, int32_t *cn) {
+    const uint64_t E = row_ptr[V];
+    uint32_t *s = (uint32_t *)malloc((E ? E : 1) * sizeof(uint32_t));
+    memcpy(s, col, E * sizeof(uint32_t));
+    for (uint32_t x = 0; x < V; ++x) qsort(s + row_ptr[x], row_ptr[x + 1] - row_ptr[x], sizeof(uint32_t), cmp_u32);
+    for (uint64_t i = 0; i < m; ++i) {
+        const uint32_t a = u[i], b = v[i];
+        uint64_t p = row_ptr[a], pe = row_ptr[a + 1], q = row_ptr[b], qe = row_ptr[b + 1];
+        int32_t c = 0;
+        int64_t last = -1;
+        while (p < pe && q < qe) {
+            if (s[p] < s[q]) ++p;
+            else if (s[p] > s[q]) ++q;
+            else {
+                const uint32_t x = s[p];
+                if (x != a && x != b && (int64_t)x != last) ++c;
+                last = x;
+                ++p;
+                ++q;
+            }
+        }
+        cn[i] = c;
+    }
+    free(s);
+}
+
 /* ---- -classes: k-means, src/word2vec.cu:2128-2175 ----------------------
  * Rows in vocabulary order (row a = syn0 row of vertex order[a]); float
  * arithmetic in the reference's order. */

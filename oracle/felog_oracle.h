@@ -156,6 +156,11 @@ uint32_t orc_eval_subset(uint32_t deg, uint32_t v, uint32_t round, uint64_t seed
 uint32_t orc_eval(uint32_t V, uint32_t d, const float *syn0, const uint64_t *row_ptr, const uint32_t *col,
                   const uint8_t *in_vocab, uint8_t *flags, float *scores, uint32_t round, uint64_t seed);
 
+/* Common neighbours (src/tools/gcn.cpp:304-357) of m pairs (u[i], v[i]):
+ * distinct vertices adjacent to both, the pair's own vertices excluded. */
+void orc_common_neighbours(uint32_t V, const uint64_t *row_ptr, const uint32_t *col, uint64_t m, const uint32_t *u,
+                           const uint32_t *v, int32_t *cn);
+
 /* -classes k-means (src/word2vec.cu:2128-2175) over the vocabulary's rows;
  * cl[vocab_size] receives the classes in vocabulary order. */
 void orc_kmeans_classes(uint32_t d, const float *syn0, const uint32_t *order, uint64_t vocab_size, uint32_t classes,

@@ -283,6 +283,15 @@ def eval_subset(deg, v, rnd, seed):
     return out[:n].copy()
 
 
+def common_neighbours(V, row_ptr, col, u, v):
+    u = np.ascontiguousarray(u, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    out = np.empty(max(len(u), 1), np.int32)
+    lib().orc_common_neighbours(C.c_uint32(V), P(np.ascontiguousarray(row_ptr, np.uint64)),
+                                P(np.ascontiguousarray(col, np.uint32)), C.c_uint64(len(u)), P(u), P(v), P(out))
+    return out[:len(u)]
+
+
 def kmeans_classes(syn0, order, vocab_size, classes, iters=10):
     """-classes k-means restated; classes of the vocabulary in vocabulary order."""
     V, d = syn0.shape
