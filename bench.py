@@ -85,11 +85,12 @@ def measured_traffic(args):
     return {name: round(v["hbm_bytes"]) for name, v in kernels.items()}
 
 
-def _auc(emb, pos, neg, orc):
-    def cos(pairs):
-        a, b = emb[pairs[:, 0]], emb[pairs[:, 1]]
-        return (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-12)
-    return orc.auc(cos(pos).astype(np.float32), cos(neg).astype(np.float32))
+def _auc(f, pos, neg):
+    """Held-out AUC on the device (felog_link_auc): cosine of syn0 rows,
+    Mann-Whitney with ties at half weight."""
+    pairs = np.concatenate([pos, neg]).astype(np.uint32)
+    labels = np.concatenate([np.ones(len(pos), np.uint8), np.zeros(len(neg), np.uint8)])
+    return f.link_auc(pairs, labels)[0]
 
 
 def train_budget_model(args):
@@ -125,8 +126,6 @@ def end_to_end(src, dst, w, n, args, model):
     both as captured rounds (felog_run_graph).  With --e2e-no-auc the whole
     graph is trained and no AUC is measured (the host split is too slow for
     the 41.6M graph)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import orc
     if args.e2e_no_auc:
         ts, td, tw, pos, neg = src, dst, w, None, None
     else:
@@ -175,8 +174,7 @@ def end_to_end(src, dst, w, n, args, model):
         vs = f.visit_stats()
         auc = None
         if pos is not None:
-            emb, _ = f.embeddings()
-            auc = _auc(emb, pos, neg, orc)
+            auc = _auc(f, pos, neg)
         print(f"[bench] e2e {form}: build {t_build * 1e3:.1f} ms + {st['rounds_done']} rounds "
               f"{(t2 - t1) * 1e3:.1f} ms, AUC {auc}", file=sys.stderr, flush=True)
         out[form] = dict(wall_ms=(t2 - t0) * 1e3, build_ms=t_build * 1e3, rounds_ms=(t2 - t1) * 1e3,
