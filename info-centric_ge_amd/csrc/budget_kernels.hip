@@ -6,7 +6,9 @@
 // Pearson stop rule, include/walk.hpp:945-951,1056-1062).  The learned
 // budget predicts both from eight per-vertex features with an 8-64-64-2 MLP
 // (ReLU), trained on the budgets the plain rule produced on other graphs:
-//   features   one wavefront per vertex over its CSR row
+//   features   one wavefront per vertex over its CSR row; rows longer than
+//              kHubChunk entries in chunks of kHubChunk (one wavefront each,
+//              partials combined per hub in chunk order: deterministic)
 //   training   mean squared error on standardised log targets; a 64-sample
 //              workgroup runs forward and backward with its activations in
 //              LDS, reduces the weight gradients over its samples and adds
@@ -63,6 +65,7 @@ __global__ __launch_bounds__(256) void k_features(const VertexRec *__restrict__ 
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < V; v += nw) {
         const VertexRec vr = vrec[v];
+        if (vr.deg > kHubChunk) continue;  // k_feat_chunk / k_feat_hub
         float ws = 0.f, wm = 0.f, nd = 0.f, ndm = 0.f;
         for (uint32_t j = lane; j < vr.deg; j += 64) {
             const float x = w[vr.begin + j];
@@ -93,6 +96,90 @@ __global__ __launch_bounds__(256) void k_features(const VertexRec *__restrict__ 
             f[4] = log1pf(nd);
             f[5] = log1pf(ndm);
             f[6] = h;
+            f[7] = in_vocab[v] ? 1.f : 0.f;
+        }
+    }
+}
+
+// Hub rows: chunk t sums (w, max w, neighbour degree, max neighbour degree,
+// w ln w) over its kHubChunk entries into part[8t ..].  The entropy of a hub
+// row is ln S - (sum w ln w) / S, so one pass over the row suffices.
+constexpr int kPart = 8;
+__global__ __launch_bounds__(256) void k_feat_chunk(const VertexRec *__restrict__ vrec,
+                                                    const uint32_t *__restrict__ col, const float *__restrict__ w,
+                                                    const uint32_t *__restrict__ hub,
+                                                    const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                    float *__restrict__ part) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nchunks = cstart[nh];
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < nchunks; t += nw) {
+        const uint32_t k = hub_of_chunk(cstart, nh, t);
+        const VertexRec vr = vrec[hub[k]];
+        const uint64_t c0 = (t - cstart[k]) * kHubChunk;
+        const uint64_t cn = vr.deg - c0 < kHubChunk ? vr.deg - c0 : kHubChunk;
+        const uint64_t b = vr.begin + c0;
+        float ws = 0.f, wm = 0.f, nd = 0.f, ndm = 0.f, wl = 0.f;
+        for (uint64_t j = lane; j < cn; j += 64) {
+            const float x = w[b + j];
+            const float du = (float)vrec[col[b + j]].deg;
+            ws += x;
+            wm = fmaxf(wm, x);
+            nd += du;
+            ndm = fmaxf(ndm, du);
+            if (x > 0.f) wl += x * __logf(x);
+        }
+        ws = wave_sum(ws);
+        wm = wave_max(wm);
+        nd = wave_sum(nd);
+        ndm = wave_max(ndm);
+        wl = wave_sum(wl);
+        if (lane == 0) {
+            float *o = part + t * kPart;
+            o[0] = ws;
+            o[1] = wm;
+            o[2] = nd;
+            o[3] = ndm;
+            o[4] = wl;
+        }
+    }
+}
+
+// One wavefront per hub: its chunk partials (lane l takes chunks l, l + 64,
+// ... in order) and the features.
+__global__ __launch_bounds__(256) void k_feat_hub(const VertexRec *__restrict__ vrec,
+                                                  const uint32_t *__restrict__ hub,
+                                                  const uint64_t *__restrict__ cstart, uint32_t nh,
+                                                  const float *__restrict__ part,
+                                                  const uint8_t *__restrict__ in_vocab, float *__restrict__ out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nh; k += nw) {
+        float ws = 0.f, wm = 0.f, nd = 0.f, ndm = 0.f, wl = 0.f;
+        for (uint64_t t = cstart[k] + lane; t < cstart[k + 1]; t += 64) {
+            const float *p = part + t * kPart;
+            ws += p[0];
+            wm = fmaxf(wm, p[1]);
+            nd += p[2];
+            ndm = fmaxf(ndm, p[3]);
+            wl += p[4];
+        }
+        ws = wave_sum(ws);
+        wm = wave_max(wm);
+        nd = wave_sum(nd);
+        ndm = wave_max(ndm);
+        wl = wave_sum(wl);
+        if (lane == 0) {
+            const uint32_t v = hub[k];
+            const uint32_t deg = vrec[v].deg;
+            float *f = out + (uint64_t)v * kF;
+            f[0] = log1pf((float)deg);
+            f[1] = log1pf(ws);
+            f[2] = wm;
+            f[3] = ws / (float)deg;
+            f[4] = log1pf(nd);
+            f[5] = log1pf(ndm);
+            f[6] = ws > 0.f ? (logf(ws) - wl / ws) / logf((float)deg) : 0.f;
             f[7] = in_vocab[v] ? 1.f : 0.f;
         }
     }
@@ -211,7 +298,7 @@ __global__ void k_adam(float *__restrict__ P, float *__restrict__ grad, float *_
 __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F, uint32_t V,
                                                      const felog_budget_model *__restrict__ M, uint32_t max_len,
                                                      uint32_t max_count, uint16_t *__restrict__ cnt,
-                                                     uint16_t *__restrict__ len) {
+                                                     uint16_t *__restrict__ len, uint32_t *__restrict__ sum) {
     __shared__ float sp[FELOG_BUDGET_PARAMS];
     __shared__ float st[6 * kF + 2 * kO];
     for (uint32_t i = threadIdx.x; i < (uint32_t)FELOG_BUDGET_PARAMS; i += blockDim.x) sp[i] = M->params[i];
@@ -227,6 +314,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F
     }
     __syncthreads();
     const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t mx = 0, n_plain = 0, n_plain_vocab = 0;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
         float x[kF];
         bool inside = true;
@@ -258,6 +346,19 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F
         uint32_t lv = (uint32_t)fminf(fmaxf(rintf(l_), 2.f), (float)max_len);
         cnt[v] = inside ? (uint16_t)cv : 0;
         len[v] = inside ? (uint16_t)lv : 0;
+        if (inside) mx = cv > mx ? cv : mx;
+        n_plain += inside ? 0u : 1u;
+        n_plain_vocab += (!inside && F[(uint64_t)v * kF + 7] != 0.f) ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+        n_plain += (uint32_t)__shfl_xor((int)n_plain, o);
+        n_plain_vocab += (uint32_t)__shfl_xor((int)n_plain_vocab, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&sum[0], mx);
+        atomicAdd(&sum[1], n_plain);
+        atomicAdd(&sum[2], n_plain_vocab);
     }
 }
 
@@ -329,6 +430,16 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap) {
 hipError_t budget_features(felog_ctx *c, float *d_out) {
     k_features<<<grid_for((uint64_t)c->V * 64, 256, 1u << 16), 256, 0, c->stream>>>(c->d_vrec, c->d_col, c->d_w,
                                                                                       c->d_in_vocab, c->V, d_out);
+    HIP_TRY(hipGetLastError());
+    const HubPlan &h = c->hubs;
+    if (h.n == 0) return hipSuccess;
+    if (!c->d_feat_part) HIP_TRY(hipMalloc(&c->d_feat_part, h.chunks * kPart * 4));
+    k_feat_chunk<<<grid_for(h.chunks * 64, 256, 1u << 16), 256, 0, c->stream>>>(c->d_vrec, c->d_col, c->d_w, h.hub,
+                                                                                 h.cstart, h.n, c->d_feat_part);
+    HIP_TRY(hipGetLastError());
+    k_feat_hub<<<grid_for((uint64_t)h.n * 64, 256, 1u << 16), 256, 0, c->stream>>>(c->d_vrec, h.hub, h.cstart, h.n,
+                                                                                    c->d_feat_part, c->d_in_vocab,
+                                                                                    d_out);
     return hipGetLastError();
 }
 
@@ -347,10 +458,11 @@ hipError_t budget_adam(felog_ctx *c, float *d_p, float *d_grad, float *d_m, floa
     return hipGetLastError();
 }
 
-hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model) {
+hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model, uint32_t *d_sum) {
+    HIP_TRY(hipMemsetAsync(d_sum, 0, 16, c->stream));
     k_mlp_predict<<<grid_for(c->V, 256, 4096), 256, 0, c->stream>>>(d_feat, c->V, d_model, c->opt.max_walk_len,
                                                                      kBudgetMaxCount, c->d_budget_cnt,
-                                                                     c->d_budget_len);
+                                                                     c->d_budget_len, d_sum);
     return hipGetLastError();
 }
 

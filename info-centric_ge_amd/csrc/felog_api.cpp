@@ -75,7 +75,7 @@ void free_all(felog_ctx *c) {
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
                     c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats,
                     c->d_w2v_ftok, c->d_w2v_kcnt, c->d_w2v_blk, c->d_w2v_table, c->d_w2v_exp, c->d_scol,
-                    c->d_cn};
+                    c->d_cn, c->hubs.hub, c->hubs.cstart, c->d_feat_part, c->d_budget_sum};
     for (void *p : ptrs) dfree(p);
     drop_round_graph(c);
     dfree(c->d_rs);
@@ -604,28 +604,19 @@ felog_status felog_budget_set(felog_ctx *c, const felog_budget_model *model) {
     if (!c->d_model) TRY(c, hipMalloc(&c->d_model, sizeof(felog_budget_model)));
     if (!c->d_budget_cnt) TRY(c, hipMalloc(&c->d_budget_cnt, (uint64_t)c->V * 2));
     if (!c->d_budget_len) TRY(c, hipMalloc(&c->d_budget_len, (uint64_t)c->V * 2));
+    if (!c->d_budget_sum) TRY(c, hipMalloc(&c->d_budget_sum, 16));
     TRY(c, hipMemcpyAsync(c->d_model, model, sizeof(felog_budget_model), hipMemcpyHostToDevice, c->stream));
     TRY(c, budget_features(c, c->d_feat));
-    TRY(c, budget_predict(c, c->d_feat, c->d_model));
-    std::vector<uint16_t> cnt(c->V);
-    TRY(c, hipMemcpyAsync(cnt.data(), c->d_budget_cnt, (uint64_t)c->V * 2, hipMemcpyDeviceToHost, c->stream));
+    TRY(c, budget_predict(c, c->d_feat, c->d_model, c->d_budget_sum));
+    uint32_t sum[4] = {0, 0, 0, 0};
+    TRY(c, hipMemcpyAsync(sum, c->d_budget_sum, 16, hipMemcpyDeviceToHost, c->stream));
     TRY(c, hipStreamSynchronize(c->stream));
-    uint32_t mx = 0;
-    bool plain_left = false;
-    for (uint16_t x : cnt) {
-        mx = x > mx ? x : mx;
-        plain_left = plain_left || x == 0;
-    }
-    c->budget_rounds = mx;
-    c->budget_plain_left = plain_left;
+    c->budget_rounds = sum[0];
+    c->budget_plain_left = sum[1] != 0;
     c->budget_on = true;
     // the plain rule's stop ratio now runs over the plain-rule vertices only:
     // its first denominator is their vocabulary count (src/word2vec.cu:1980)
-    std::vector<uint8_t> inv(c->V);
-    TRY(c, hipMemcpy(inv.data(), c->d_in_vocab, c->V, hipMemcpyDeviceToHost));
-    uint64_t plain_vocab = 0;
-    for (uint32_t v = 0; v < c->V; ++v) plain_vocab += (inv[v] && cnt[v] == 0) ? 1 : 0;
-    c->last_eva_num = (uint32_t)plain_vocab;
+    c->last_eva_num = sum[2];
     return FELOG_OK;
 }
 

@@ -175,4 +175,16 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
     return (uint16_t)(b >> 16);
 }
 
+// Chunk plans over long rows: chunk t belongs to the row k with
+// cstart[k] <= t < cstart[k + 1] (nh rows, cstart ascending).
+__device__ __forceinline__ uint32_t hub_of_chunk(const uint64_t *__restrict__ cstart, uint32_t nh, uint64_t t) {
+    uint32_t lo = 0, hi = nh;  // last k with cstart[k] <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (cstart[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 }  // namespace felog

@@ -26,6 +26,17 @@ struct EdgeRec {
 };
 static_assert(sizeof(EdgeRec) == 32, "one 32-byte gather per walk step");
 
+// Rows longer than kHubChunk CSR entries ("hubs") are processed in chunks of
+// kHubChunk entries by one wavefront each; hub k (vertex hub[k]) owns chunks
+// cstart[k] .. cstart[k + 1] - 1 of the plan.
+constexpr uint32_t kHubChunk = 4096;
+struct HubPlan {
+    uint32_t *hub = nullptr;     // n
+    uint64_t *cstart = nullptr;  // n + 1 (device)
+    uint32_t n = 0;
+    uint64_t chunks = 0;
+};
+
 struct Corpus {
     uint32_t round = 0;
     uint64_t n_walks = 0, n_tokens = 0, n_steps = 0;
@@ -94,6 +105,7 @@ struct felog_ctx {
     felog::VertexRec *d_vwalk = nullptr; // V: vrec with the kSelfFirst bit
     felog::EdgeRec *d_erec = nullptr;    // E
     uint32_t *d_deg = nullptr;           // V
+    felog::HubPlan hubs;                 // rows of more than kHubChunk entries
 
     // vocabulary / sampling tables
     uint32_t *d_vocab_order = nullptr;   // V, rank -> vertex
@@ -156,6 +168,8 @@ struct felog_ctx {
     uint16_t *d_budget_cnt = nullptr, *d_budget_len = nullptr;  // V
     felog_budget_model *d_model = nullptr;
     float *d_feat = nullptr;             // V x FELOG_BUDGET_FEATURES
+    float *d_feat_part = nullptr;        // hub chunk partials of the features
+    uint32_t *d_budget_sum = nullptr;    // prediction summary (4 words)
     uint32_t budget_rounds = 0;          // largest predicted count
     bool budget_plain_left = false;      // some vertex keeps the plain rule (count 0)
     bool budget_recheck = false;         // score budgeted vertices once at min_R
@@ -280,7 +294,10 @@ hipError_t budget_features(felog_ctx *c, float *d_out);
 hipError_t budget_grad(felog_ctx *c, const float *d_x, const float *d_t, uint64_t n, uint64_t sample_seed,
                        uint32_t step, uint32_t blocks, const float *d_p, float *d_grad, double *d_loss, float inv_n);
 hipError_t budget_adam(felog_ctx *c, float *d_p, float *d_grad, float *d_m, float *d_v, float lr, uint32_t t);
-hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model);
+// Counts and lengths of every vertex; d_sum (4 words, zeroed here) receives
+// the largest count, the vertices left to the plain rule (count 0) and those
+// of them in the vocabulary.
+hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model, uint32_t *d_sum);
 // start flags (0/1 per walker) into d_ok
 hipError_t budget_walk_ok(felog_ctx *c, uint32_t *d_ok);
 hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok);

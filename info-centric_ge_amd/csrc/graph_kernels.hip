@@ -86,17 +86,6 @@ __global__ void k_vwalk(const VertexRec *__restrict__ vrec, const uint32_t *__re
     }
 }
 
-// Both neighbours' walk records into every edge record.
-__global__ void k_edge_vr(const VertexRec *__restrict__ vwalk, uint64_t E, EdgeRec *__restrict__ erec) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += stride) {
-        EdgeRec r = erec[e];
-        r.col_vr = vwalk[r.col];
-        r.alias_vr = vwalk[r.alias_col];
-        erec[e] = r;
-    }
-}
-
 // syn0 initialisation of src/word2vec.cu:893-896: element e of the
 // vocabulary-ordered table takes x_{e+1} of the LCG x <- x*25214903917 + 11
 // started at 1, as ((x & 0xFFFF)/65536 - 0.5)/d.  jump[k] holds the affine
@@ -110,21 +99,38 @@ __device__ __forceinline__ unsigned long long lcg_nth(const Affine *__restrict__
     return x;
 }
 
+// One thread per 8 consecutive elements of a row (dp is a multiple of 32):
+// one jump to the first element's LCG state, then single steps.
 template <typename T>
 __global__ void k_init_syn0(const uint32_t *__restrict__ rank, uint32_t V, uint32_t d, uint32_t dp,
                             const Affine *__restrict__ jump, T *__restrict__ syn0) {
-    const uint64_t total = (uint64_t)V * dp;
+    const uint32_t per_row = dp / 8;
+    const uint64_t total = (uint64_t)V * per_row;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint32_t v = (uint32_t)(i / dp), b = (uint32_t)(i % dp);
-        float val = 0.f;
-        if (b < d) {
-            const uint64_t e = (uint64_t)rank[v] * d + b;
-            const unsigned long long x = lcg_nth(jump, e + 1);
-            val = (float)(((double)(x & 0xFFFFull) / 65536.0 - 0.5) / (double)d);
+        const uint32_t v = (uint32_t)(i / per_row), b0 = (uint32_t)(i % per_row) * 8;
+        float val[8];
+        unsigned long long x = 0;
+        if (b0 < d) x = lcg_nth(jump, (uint64_t)rank[v] * d + b0 + 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            val[k] = 0.f;
+            if (b0 + k < d) {
+                val[k] = (float)(((double)(x & 0xFFFFull) / 65536.0 - 0.5) / (double)d);
+                x = x * 25214903917ull + 11ull;
+            }
         }
-        if constexpr (sizeof(T) == 4) syn0[i] = val;
-        else syn0[i] = f32_to_bf16_rne(val);
+        T *row = syn0 + (uint64_t)v * dp + b0;
+        if constexpr (sizeof(T) == 4) {
+            reinterpret_cast<float4 *>(row)[0] = make_float4(val[0], val[1], val[2], val[3]);
+            reinterpret_cast<float4 *>(row)[1] = make_float4(val[4], val[5], val[6], val[7]);
+        } else {
+            uint32_t p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                p[k] = (uint32_t)f32_to_bf16_rne(val[2 * k]) | ((uint32_t)f32_to_bf16_rne(val[2 * k + 1]) << 16);
+            reinterpret_cast<uint4 *>(row)[0] = make_uint4(p[0], p[1], p[2], p[3]);
+        }
     }
 }
 
@@ -164,7 +170,7 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap) {
 // ---------------------------------------------------------------------------
 constexpr uint64_t kUnit = 1ull << 32;
 constexpr uint64_t kHeavyBit = 1ull << 63;
-constexpr uint32_t kChunk = 4096;
+constexpr uint32_t kChunk = kHubChunk;
 
 __device__ __forceinline__ double wclamp(float x) { return x > 0.f ? (double)x : 0.0; }
 __device__ __forceinline__ double wclamp(double x) { return x > 0.0 ? x : 0.0; }
@@ -244,17 +250,7 @@ __global__ __launch_bounds__(256) void k_alias_small(const uint64_t *__restrict_
 
 // Larger segments ("hubs", hub k = segment hub[k]) are cut into chunks;
 // chunk t of the launch belongs to the hub k with cstart[k] <= t <
-// cstart[k+1].
-__device__ __forceinline__ uint32_t hub_of_chunk(const uint64_t *__restrict__ cstart, uint32_t nh, uint64_t t) {
-    uint32_t lo = 0, hi = nh;  // last k with cstart[k] <= t
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (cstart[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
+// cstart[k+1] (hub_of_chunk).
 template <typename W>
 __global__ __launch_bounds__(256) void k_alias_chunk_sum(const uint64_t *__restrict__ seg,
                                                          const uint32_t *__restrict__ hub,
@@ -439,10 +435,12 @@ __global__ __launch_bounds__(256) void k_alias_pair(const uint64_t *__restrict__
 struct EdgeOut {
     const uint64_t *seg;
     const uint32_t *col;
+    const VertexRec *vwalk;
     EdgeRec *erec;
     __device__ void operator()(uint64_t e, uint64_t j, uint32_t thr, uint64_t a) const {
         const uint64_t b = e - j;
-        erec[e] = EdgeRec{thr, col[e], col[b + a], (uint32_t)a, {}, {}};
+        const uint32_t c0 = col[e], c1 = col[b + a];
+        erec[e] = EdgeRec{thr, c0, c1, (uint32_t)a, vwalk[c0], vwalk[c1]};
     }
 };
 struct PairOut {
@@ -570,10 +568,11 @@ hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *ds
 // Exact alias tables of nseg weight segments: seg holds nseg + 1 column
 // offsets on the device (total = the last one), w the weights, seg_of the
 // segment of every column (null when nseg == 1); out stores each column's
-// threshold and alias.  Temporary device memory: 28 bytes per column.
+// threshold and alias.  Temporary device memory: 28 bytes per column.  keep
+// (when set) receives the chunk plan of the segments longer than one chunk.
 template <typename W, typename Out>
 static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nseg, uint64_t total, const W *w,
-                                 const uint32_t *seg_of, Out out) {
+                                 const uint32_t *seg_of, Out out, HubPlan *keep) {
     if (total == 0 || nseg == 0) return hipSuccess;
     hipStream_t st = c->stream;
     uint64_t *val = nullptr, *LD = nullptr, *SH = nullptr;
@@ -608,6 +607,7 @@ static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nse
     HIP_TRY(hipStreamSynchronize(st));
     uint64_t *cnt = nullptr, *cstart = nullptr, *cq = nullptr, *hrem = nullptr;
     double *csum = nullptr, *hsum = nullptr;
+    uint64_t nchunks = 0;
     if (nh) {
         HIP_TRY(hipMalloc(&cnt, ((uint64_t)nh + 1) * 8));
         HIP_TRY(hipMalloc(&cstart, ((uint64_t)nh + 1) * 8));
@@ -615,7 +615,6 @@ static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nse
         HIP_TRY(hipGetLastError());
         tb = tmax;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, cstart, (uint64_t)nh + 1, st));
-        uint64_t nchunks = 0;
         HIP_TRY(hipMemcpyAsync(&nchunks, cstart + nh, 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         HIP_TRY(hipMalloc(&csum, nchunks * 8));
@@ -644,6 +643,11 @@ static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nse
     k_alias_pair<Out><<<grid_for(total, 256, 1u << 20), 256, 0, st>>>(seg, seg_of, total, val, LD, SH, NL, out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
+    if (keep && nh) {
+        *keep = HubPlan{hub, cstart, nh, nchunks};
+        hub = nullptr;
+        cstart = nullptr;
+    }
     for (void *p : {(void *)val, (void *)LD, (void *)SH, (void *)NL, (void *)hub, (void *)d_nh, tmp, (void *)cnt,
                     (void *)cstart, (void *)cq, (void *)hrem, (void *)csum, (void *)hsum})
         dfree(p);
@@ -699,19 +703,16 @@ hipError_t build_csr(felog_ctx *c, uint32_t **edge_src) {
     return hipSuccess;
 }
 
+// Edge records (alias columns and both neighbours' walk records, written
+// once per entry by the pairing kernel).
 hipError_t build_alias(felog_ctx *c, const uint32_t *edge_src) {
     const uint64_t E = c->E;
-    HIP_TRY(hipMalloc(&c->d_erec, E * sizeof(EdgeRec) + 16));
-    HIP_TRY(alias_segments(c, c->d_row_ptr, c->V, E, (const float *)c->d_w, edge_src,
-                           EdgeOut{c->d_row_ptr, c->d_col, c->d_erec}));
     HIP_TRY(hipMalloc(&c->d_vwalk, (uint64_t)c->V * sizeof(VertexRec) + 8));
     k_vwalk<<<grid_for(c->V, 256, 65535), 256, 0, c->stream>>>(c->d_vrec, c->d_col, c->V, c->d_vwalk);
     HIP_TRY(hipGetLastError());
-    if (E) {
-        k_edge_vr<<<grid_for(E, 256, 1u << 16), 256, 0, c->stream>>>(c->d_vwalk, E, c->d_erec);
-        HIP_TRY(hipGetLastError());
-    }
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMalloc(&c->d_erec, E * sizeof(EdgeRec) + 16));
+    HIP_TRY(alias_segments(c, c->d_row_ptr, c->V, E, (const float *)c->d_w, edge_src,
+                           EdgeOut{c->d_row_ptr, c->d_col, c->d_vwalk, c->d_erec}, &c->hubs));
     return hipSuccess;
 }
 
@@ -757,7 +758,7 @@ hipError_t build_vocab_and_tables(felog_ctx *c) {
     HIP_TRY(hipGetLastError());
     uint64_t h_tot[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(h_tot, totals, 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(alias_segments(c, seg01, 1, V, (const double *)uw, nullptr, PairOut{c->d_uni}));
+    HIP_TRY(alias_segments(c, seg01, 1, V, (const double *)uw, nullptr, PairOut{c->d_uni}, nullptr));
     c->vocab_size = h_tot[0];
     c->train_words = h_tot[1];
     for (void *p : {(void *)keys_a, (void *)keys_b, (void *)ids_a, (void *)ids_b, (void *)totals, (void *)seg01,
@@ -795,10 +796,10 @@ hipError_t init_tables(felog_ctx *c) {
     HIP_TRY(hipMalloc(&d_jump, sizeof(jump)));
     HIP_TRY(hipMemcpyAsync(d_jump, jump, sizeof(jump), hipMemcpyHostToDevice, c->stream));
     if (es == 4)
-        k_init_syn0<float><<<grid_for(n, 256, 1u << 20), 256, 0, c->stream>>>(
+        k_init_syn0<float><<<grid_for(n / 8, 256, 1u << 20), 256, 0, c->stream>>>(
             c->d_vocab_rank, c->V, c->d, c->dp, d_jump, (float *)c->d_syn0);
     else
-        k_init_syn0<uint16_t><<<grid_for(n, 256, 1u << 20), 256, 0, c->stream>>>(
+        k_init_syn0<uint16_t><<<grid_for(n / 8, 256, 1u << 20), 256, 0, c->stream>>>(
             c->d_vocab_rank, c->V, c->d, c->dp, d_jump, (uint16_t *)c->d_syn0);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(c->d_syn1, 0, n * es, c->stream));

@@ -230,7 +230,9 @@ template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sgns_tile(Params a) {
     constexpr uint32_t GT = gt_pitch(sizeof(T), RM);  // pitch of the count / G^T tile (words)
     constexpr uint32_t EPV = 16 / sizeof(T);          // elements per 16-byte vector
-    constexpr int NB = sizeof(T) == 4 ? 8 : 5;        // gathered vectors held per thread
+    // gathered vectors held per thread: all of a 32 + 32-row bf16 tile at
+    // d = 128 (4), else the first NB of the tile's vectors
+    constexpr int NB = sizeof(T) == 4 ? 8 : ((DP == 128 && RM == 32) ? 4 : 5);
     extern __shared__ __align__(16) unsigned char lds[];
     const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM, a.negative);
     const uint32_t ldv = L.ldv;
@@ -589,27 +591,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
                 const uint32_t o = ot * 16 + l16;
                 const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
-                // G^T[o][r], r = 32*kb + 8*grp + s: two 4-row blocks
-                bf8 ghi[2], glo[2];
-#pragma unroll
-                for (int kb = 0; kb < 2; ++kb) {
-                    const uint4 *gp = (const uint4 *)(lds + gb + (o * GT + 32 * kb + 8 * grp) * 4);
-                    const uint4 x = (uint32_t)kb < nk32 ? gp[0] : make_uint4(0, 0, 0, 0);
-                    const uint4 y = (uint32_t)kb < nk32 ? gp[1] : make_uint4(0, 0, 0, 0);
-                    ghi[kb] = __builtin_bit_cast(bf8, u4v{x.x, x.y, y.x, y.y});
-                    glo[kb] = __builtin_bit_cast(bf8, u4v{x.z, x.w, y.z, y.w});
-                }
                 for (uint32_t dt = wave; dt < dtiles; dt += 4) {
                     const uint32_t col0 = dt * 16 + g4;
                     f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int kb = 0; kb < 2; ++kb) {
                         if ((uint32_t)kb >= nk32) break;
+                        // G^T[o][r], r = 32*kb + 8*grp + s: two 4-row blocks
+                        // (read per column block: fewer live registers)
+                        const uint4 *gp = (const uint4 *)(lds + gb + (o * GT + 32 * kb + 8 * grp) * 4);
+                        const uint4 x = gp[0], y = gp[1];
+                        const bf8 ghi = __builtin_bit_cast(bf8, u4v{x.x, x.y, y.x, y.y});
+                        const bf8 glo = __builtin_bit_cast(bf8, u4v{x.z, x.w, y.z, y.w});
                         // Vin[r][16*dt + lane%16], transposed from the staged rows
                         const uint32_t b0 = rb + (32 * kb + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
                         const bf8 vv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, ghi[kb], acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, glo[kb], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, ghi, acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, glo, acc, 0, 0, 0);
                     }
                     if constexpr (DBG) {
                         if (dbg_on)

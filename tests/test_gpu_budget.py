@@ -44,6 +44,54 @@ def test_features_match_oracle(er):
     assert np.allclose(got, ref, rtol=1e-5, atol=1e-5), np.abs(got - ref).max(0)
 
 
+def test_hub_features_match_oracle():
+    """Rows longer than one 4096-entry chunk take the chunked path: two hubs
+    of 9000 and 5000 neighbours (3 and 2 chunks) on a random graph, with
+    the budget summary the prediction reports (largest count, vertices left
+    to the plain rule)."""
+    rng = np.random.default_rng(11)
+    n = 12_000
+    s0, d0, w0 = graphgen.erdos_renyi(n, 30_000, seed=8, weights="uniform")
+    h0 = rng.choice(np.arange(2, n), 9000, replace=False)
+    h1 = rng.choice(np.arange(2, n), 5000, replace=False)
+    src = np.concatenate([s0, np.zeros(9000, np.uint32), np.ones(5000, np.uint32)]).astype(np.uint32)
+    dst = np.concatenate([d0, h0, h1]).astype(np.uint32)
+    key = np.minimum(src, dst).astype(np.int64) * n + np.maximum(src, dst)
+    _, first = np.unique(key, return_index=True)
+    first.sort()
+    src, dst = src[first], dst[first]
+    w = rng.uniform(1e-3, 1.0, len(src)).astype(np.float32)
+    f = ctx(src, dst, w, n)
+    try:
+        got = f.budget_features()
+        rp, col, wc = f.csr()
+        deg = np.diff(rp.astype(np.int64))
+        assert deg[0] > 2 * 4096 and 4096 < deg[1] <= 2 * 4096
+        order, vdeg = f.vocab()
+        inv = (vdeg >= 5).astype(np.uint8)
+        inv[order[0]] = 1
+        ref = budget.features(rp, col, wc, inv)
+        err = np.abs(got - ref).max(0)
+        print(f"feature errors {err}, hubs {got[:2]}")
+        assert np.allclose(got, ref, rtol=1e-5, atol=1e-5), err
+        # the prediction's summary drives the budgeted run: every vertex
+        # budgeted with count 3 -> exactly 3 rounds, no plain-rule rounds
+        f.budget_set(walk_budget.constant_model(3, 7))
+        cnt, _, rounds = f.budget()
+        assert rounds == 3 and (cnt == 3).all()
+        f.run_budget()
+        assert f.stats()["rounds_done"] == 3
+        # a box on ln(1 + degree) leaves the two hubs to the plain rule
+        lo, hi = np.full(8, -1e30), np.full(8, 1e30)
+        hi[0] = np.log1p(4096.0)
+        f.budget_set(walk_budget.constant_model(3, 7, box=(lo, hi)))
+        cnt, _, rounds = f.budget()
+        assert rounds == 3 and (cnt[:2] == 0).all() and (cnt[2:] == 3).all()
+    finally:
+        f.budget_set(None)
+        f.close()
+
+
 def test_mlp_gradient_matches_fp64():
     """The HIP forward/backward over 1000 samples against FP64 backprop:
     norm-wise relative error of the gradient below 1e-5 (f32 arithmetic,
