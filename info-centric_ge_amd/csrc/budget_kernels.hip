@@ -375,25 +375,44 @@ __global__ void k_walk_ok_budget(const uint8_t *__restrict__ flags, const uint16
     }
 }
 
-// n_dev (when set) holds the token count instead of n.
-__global__ void k_visits(const uint32_t *__restrict__ tok, uint64_t n, const uint64_t *__restrict__ n_dev,
-                         uint32_t *__restrict__ visits) {
+// Visit counts, indexed by vocabulary rank (vertices by degree, descending).
+// A walk visits a vertex about in proportion to its degree, so the first
+// kVisitHubs ranks take most tokens: those are counted in an LDS histogram
+// per workgroup and added once at its end (global atomics on the same few
+// addresses serialise), the rest go straight to memory.  n_dev (when set)
+// holds the token count instead of n.
+constexpr uint32_t kVisitHubs = 12288;
+constexpr uint32_t kVisitBlock = 1024;
+__global__ __launch_bounds__(kVisitBlock) void k_visits(const uint32_t *__restrict__ tok, uint64_t n,
+                                                        const uint64_t *__restrict__ n_dev,
+                                                        const uint32_t *__restrict__ rank,
+                                                        uint32_t *__restrict__ visits) {
+    __shared__ uint32_t hub[kVisitHubs];
+    for (uint32_t i = threadIdx.x; i < kVisitHubs; i += blockDim.x) hub[i] = 0;
+    __syncthreads();
     if (n_dev) n = *n_dev;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        atomicAdd(&visits[tok[i]], 1u);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t r = rank[tok[i]];
+        if (r < kVisitHubs) atomicAdd(&hub[r], 1u);
+        else atomicAdd(&visits[r], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kVisitHubs; i += blockDim.x)
+        if (hub[i]) atomicAdd(&visits[i], hub[i]);
 }
 
 // out[0] += sum_i p_i (ln p_i - ln q_i) over vertices with p_i, q_i > 0
 // (p = degree share, q = visit share); out[1] += -sum q ln q;
 // out[2] += vertices with p_i > 0 and no visit.
 __global__ __launch_bounds__(256) void k_visit_stats(const uint32_t *__restrict__ visits,
+                                                     const uint32_t *__restrict__ rank,
                                                      const uint32_t *__restrict__ deg, uint32_t V, double inv_vis,
                                                      double inv_deg, double *__restrict__ out) {
     double kl = 0.0, ent = 0.0, unv = 0.0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        const double p = (double)deg[i] * inv_deg, q = (double)visits[i] * inv_vis;
+        const double p = (double)deg[i] * inv_deg, q = (double)visits[rank[i]] * inv_vis;
         if (q > 0.0) ent -= q * log(q);
         if (p > 0.0) {
             if (q > 0.0) kl += p * (log(p) - log(q));
@@ -475,7 +494,8 @@ hipError_t budget_walk_ok(felog_ctx *c, uint32_t *d_ok) {
 
 hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok) {
     if (n_tok == 0) return hipSuccess;
-    k_visits<<<grid_for(n_tok, 256, 1u << 16), 256, 0, c->stream>>>(d_tok, n_tok, nullptr, c->d_visits);
+    k_visits<<<grid_for(n_tok, kVisitBlock, c->cu_count * 2), kVisitBlock, 0, c->stream>>>(
+        d_tok, n_tok, nullptr, c->d_vocab_rank, c->d_visits);
     HIP_TRY(hipGetLastError());
     c->visits_total += n_tok;
     return hipSuccess;
@@ -483,7 +503,8 @@ hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok) {
 
 // The caller adds the round's token count to visits_total once it is known.
 hipError_t add_round_visits(felog_ctx *c) {
-    k_visits<<<c->cu_count * 8, 256, 0, c->stream>>>(c->d_tok, 0, &c->d_rs->n_tokens, c->d_visits);
+    k_visits<<<c->cu_count * 2, kVisitBlock, 0, c->stream>>>(c->d_tok, 0, &c->d_rs->n_tokens, c->d_vocab_rank,
+                                                            c->d_visits);
     return hipGetLastError();
 }
 
@@ -491,7 +512,8 @@ hipError_t visit_stats(felog_ctx *c, double *d_out) {
     HIP_TRY(hipMemsetAsync(d_out, 0, 3 * sizeof(double), c->stream));
     const double inv_vis = c->visits_total ? 1.0 / (double)c->visits_total : 0.0;
     const double inv_deg = c->E ? 1.0 / (double)c->E : 0.0;
-    k_visit_stats<<<grid_for(c->V, 256, 4096), 256, 0, c->stream>>>(c->d_visits, c->d_deg, c->V, inv_vis, inv_deg,
+    k_visit_stats<<<grid_for(c->V, 256, 4096), 256, 0, c->stream>>>(c->d_visits, c->d_vocab_rank, c->d_deg, c->V,
+                                                                     inv_vis, inv_deg,
                                                                      d_out);
     return hipGetLastError();
 }

@@ -174,7 +174,7 @@ struct felog_ctx {
     bool budget_plain_left = false;      // some vertex keeps the plain rule (count 0)
     bool budget_recheck = false;         // score budgeted vertices once at min_R
     // corpus visit histogram (cumulative over walk rounds while tracked)
-    uint32_t *d_visits = nullptr;        // V
+    uint32_t *d_visits = nullptr;        // V, indexed by vocabulary rank
     uint64_t visits_total = 0;
     double *d_vstats = nullptr;          // 3 doubles
 
