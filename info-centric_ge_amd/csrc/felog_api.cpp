@@ -263,13 +263,20 @@ felog_status felog_build(felog_ctx *c) {
         TRY(c, hipMalloc(&c->d_edst, 4));
         TRY(c, hipMalloc(&c->d_ew, 4));
     }
+    // the build's temporaries come from one block (falls back to separate
+    // allocations past it)
+    Scratch sc;
+    hipError_t be = sc.reserve(build_scratch_bytes(c));
     uint32_t *edge_src = nullptr;
-    hipError_t be = build_csr(c, &edge_src);
-    if (be == hipSuccess) be = build_vocab_and_tables(c);
-    if (be == hipSuccess) be = build_alias(c, edge_src);
-    dfree(edge_src);
+    if (be == hipSuccess) be = build_csr(c, sc, &edge_src);
+    if (be == hipSuccess) be = build_vocab_and_tables(c, sc);
+    if (be == hipSuccess) be = build_alias(c, sc, edge_src);
+    if (be == hipSuccess) be = init_tables(c, &sc);
+    // the sorted sources were either in the scratch or the edge input
+    dfree(c->d_esrc);
+    c->d_esrc = nullptr;
+    sc.done();
     TRY(c, be);
-    TRY(c, init_tables(c));
     const uint32_t V = c->V;
     TRY(c, hipMalloc(&c->d_flags, V));
     TRY(c, hipMemset(c->d_flags, 0, V));

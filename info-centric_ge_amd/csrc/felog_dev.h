@@ -41,7 +41,7 @@ enum : uint32_t {
     // (bucket, coin)
     DOM_SGNS = 0x10000000u,
     DOM_EVAL = 0x40000000u,       // y |= draw index, x = vertex, z = ~0
-    DOM_ROUND_BF16 = 0x50000000u, // x = thread, z = tile slot (seeds xorshift32)
+    DOM_ROUND_BF16 = 0x50000000u, // x = 0, z = tile slot: one draw per tile, hashed per thread
     DOM_MLP = 0x60000000u,        // x = sample slot of a budget-MLP step, w = step
 };
 
@@ -158,21 +158,46 @@ __device__ __forceinline__ bool cn_trace(uint32_t dv, uint32_t dx, float data, d
 }
 
 // bf16 helpers (round-to-nearest-even is never used for table updates; they
-// are stochastically rounded).
+// are stochastically rounded).  gfx950 converts in hardware:
+// v_cvt_pk_bf16_f32 rounds two values to nearest even, v_cvt_sr_bf16_f32
+// adds the high 16 bits of its random word to the value's bit pattern and
+// keeps the upper half.  On finite values both equal the integer forms
+// (b + 0x7FFF + ((b >> 16) & 1)) >> 16 and (b + r16) >> 16; NaN comes back
+// quiet.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
-__device__ __forceinline__ uint16_t f32_to_bf16_sr(float x, uint32_t r16) {
-    uint32_t b = __float_as_uint(x);
-    if ((b & 0x7F800000u) == 0x7F800000u) return (uint16_t)(b >> 16);  // inf/nan
-    b += (r16 & 0xFFFFu);
-    return (uint16_t)(b >> 16);
+// (a, b) -> the low and high halves of one word
+__device__ __forceinline__ uint32_t f32x2_to_bf16x2_rne(float a, float b) {
+    bf16x2_t v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
 }
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
-    uint32_t b = __float_as_uint(x);
-    if ((b & 0x7F800000u) == 0x7F800000u) return (uint16_t)(b >> 16);
-    b += 0x7FFFu + ((b >> 16) & 1u);
-    return (uint16_t)(b >> 16);
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+// a rounded with the high 16 bits of r, b with the low 16 bits
+__device__ __forceinline__ uint32_t f32x2_to_bf16x2_sr(float a, float b, uint32_t r) {
+    bf16x2_t v = __builtin_bit_cast(bf16x2_t, 0u);
+    v = __builtin_amdgcn_cvt_sr_bf16_f32(v, a, r, false);
+    v = __builtin_amdgcn_cvt_sr_bf16_f32(v, b, r << 16, true);
+    return __builtin_bit_cast(uint32_t, v);
+}
+// x rounded with the low 16 bits of r16
+__device__ __forceinline__ uint16_t f32_to_bf16_sr(float x, uint32_t r16) {
+    const bf16x2_t v = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(bf16x2_t, 0u), x, r16 << 16, false);
+    return (uint16_t)__builtin_bit_cast(uint32_t, v);
+}
+
+// 32-bit integer hash "lowbias32" (C. Wellons, "Hash function prospector",
+// 2018): seeds independent per-thread streams from one draw.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
 }
 
 // Chunk plans over long rows: chunk t belongs to the row k with

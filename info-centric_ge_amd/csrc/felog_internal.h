@@ -216,14 +216,41 @@ enum : int {
     kNumCounters = 18
 };
 
+// Device memory for the build's temporaries: one block taken at the start of
+// felog_build and handed out in 256-byte aligned pieces, so the build does
+// not map and unmap gigabytes of temporaries stage by stage.  Pieces are
+// released in stack order (release(mark)); a request the block cannot hold
+// falls back to hipMalloc and is freed by done().  Every user of a piece
+// runs on the context's stream, so a released piece is reused only by
+// later work on that stream.
+struct Scratch {
+    char *base = nullptr;
+    size_t cap = 0, top = 0;
+    std::vector<void *> extra;
+    hipError_t reserve(size_t bytes);
+    hipError_t take(void **p, size_t bytes);
+    template <typename T>
+    hipError_t take(T **p, size_t bytes) {
+        void *q = nullptr;
+        const hipError_t e = take(&q, bytes);
+        *p = (T *)q;
+        return e;
+    }
+    size_t mark() const { return top; }
+    void release(size_t m) { top = m; }
+    void done();
+};
+
 // graph_kernels.hip
 hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w,
                            uint64_t m, bool undirected);
+// Bytes of scratch the build takes at its peak (edge list of c->E entries).
+size_t build_scratch_bytes(felog_ctx *c);
 // CSR; *edge_src receives the vertex of every CSR entry (device, E
-// entries, owned by the caller) for the alias build.
-hipError_t build_csr(felog_ctx *c, uint32_t **edge_src);
-hipError_t build_alias(felog_ctx *c, const uint32_t *edge_src);
-hipError_t build_vocab_and_tables(felog_ctx *c);
+// entries, in sc or in the context's edge input) for the alias build.
+hipError_t build_csr(felog_ctx *c, Scratch &sc, uint32_t **edge_src);
+hipError_t build_alias(felog_ctx *c, Scratch &sc, const uint32_t *edge_src);
+hipError_t build_vocab_and_tables(felog_ctx *c, Scratch &sc);
 // walk_kernel.hip
 // Round state: eager calls set the round, its learning rate (alpha < 0: the
 // schedule's) and, unless ~0u, the sentence count; captured rounds advance
@@ -304,7 +331,7 @@ hipError_t add_visits(felog_ctx *c, const uint32_t *d_tok, uint64_t n_tok);
 // the round's corpus (token count from the round state)
 hipError_t add_round_visits(felog_ctx *c);
 hipError_t visit_stats(felog_ctx *c, double *d_out);
-hipError_t init_tables(felog_ctx *c);
+hipError_t init_tables(felog_ctx *c, Scratch *sc);
 
 uint32_t sgns_group_centres(uint32_t batch, uint32_t window, uint32_t negative);
 // Drop the captured round (its buffers are about to change).

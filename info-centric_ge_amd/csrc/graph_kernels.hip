@@ -128,7 +128,7 @@ __global__ void k_init_syn0(const uint32_t *__restrict__ rank, uint32_t V, uint3
             uint32_t p[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                p[k] = (uint32_t)f32_to_bf16_rne(val[2 * k]) | ((uint32_t)f32_to_bf16_rne(val[2 * k + 1]) << 16);
+                p[k] = f32x2_to_bf16x2_rne(val[2 * k], val[2 * k + 1]);
             reinterpret_cast<uint4 *>(row)[0] = make_uint4(p[0], p[1], p[2], p[3]);
         }
     }
@@ -538,6 +538,69 @@ __global__ void k_row_ptr(const uint32_t *__restrict__ key, uint64_t E, uint32_t
         if (e_ != hipSuccess) return e_;        \
     } while (0)
 
+hipError_t Scratch::reserve(size_t bytes) {
+    const hipError_t e = hipMalloc(&base, bytes);
+    if (e != hipSuccess) {
+        base = nullptr;
+        return e;
+    }
+    cap = bytes;
+    top = 0;
+    return hipSuccess;
+}
+
+hipError_t Scratch::take(void **p, size_t bytes) {
+    const size_t o = (top + 255) & ~(size_t)255;
+    if (base && o + bytes <= cap) {
+        *p = base + o;
+        top = o + bytes;
+        return hipSuccess;
+    }
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) extra.push_back(*p);
+    return e;
+}
+
+void Scratch::done() {
+    for (void *p : extra) dfree(p);
+    extra.clear();
+    dfree(base);
+    base = nullptr;
+    cap = top = 0;
+}
+
+// The largest of the three stages: CSR (second key buffer, two value
+// buffers, sort temporary), vocabulary (the sorted sources kept, the
+// vocabulary sort and the sampler's alias build over V) and the edge alias
+// build (the sorted sources, 28 bytes per entry, scan temporaries and chunk
+// plans, which the slack covers).
+size_t build_scratch_bytes(felog_ctx *c) {
+    const uint64_t E = c->E, V = c->V;
+    auto al = [](uint64_t b) -> size_t { return (size_t)((b + 255) & ~(uint64_t)255); };
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < V) ++bits;
+    size_t sort_e = 0, sort_v = 0;
+    {
+        hipcub::DoubleBuffer<uint32_t> k(nullptr, nullptr);
+        hipcub::DoubleBuffer<uint64_t> v(nullptr, nullptr);
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_e, k, v, E, 0, bits, c->stream) != hipSuccess)
+            sort_e = 0;
+    }
+    {
+        hipcub::DoubleBuffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_v, k, v, V, 0, 32, c->stream) != hipSuccess)
+            sort_v = 0;
+    }
+    const size_t keys = al(4 * E + 4);
+    const size_t slack_e = al(E / 8) + ((size_t)64 << 20), slack_v = al(V / 8) + ((size_t)16 << 20);
+    const size_t csr = keys + 2 * al(8 * E + 8) + al(sort_e + 16);
+    const size_t vocab = keys + 4 * al(4 * V) + 512 + al(8 * V) + al(sort_v + 16) + 3 * al(8 * V) + 2 * al(4 * V) +
+                         slack_v;
+    const size_t alias = keys + 3 * al(8 * E) + al(4 * E) + al(4 * V) + slack_e;
+    size_t m = csr > vocab ? csr : vocab;
+    return m > alias ? m : alias;
+}
+
 hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w,
                            uint64_t m, bool undirected) {
     const uint64_t E = undirected ? 2 * m : m;
@@ -571,18 +634,21 @@ hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *ds
 // threshold and alias.  Temporary device memory: 28 bytes per column.  keep
 // (when set) receives the chunk plan of the segments longer than one chunk.
 template <typename W, typename Out>
-static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nseg, uint64_t total, const W *w,
-                                 const uint32_t *seg_of, Out out, HubPlan *keep) {
+static hipError_t alias_segments(felog_ctx *c, Scratch &sc, const uint64_t *seg, uint32_t nseg, uint64_t total,
+                                 const W *w, const uint32_t *seg_of, Out out, HubPlan *keep) {
     if (total == 0 || nseg == 0) return hipSuccess;
     hipStream_t st = c->stream;
+    const size_t m0 = sc.mark();
     uint64_t *val = nullptr, *LD = nullptr, *SH = nullptr;
     uint32_t *NL = nullptr, *hub = nullptr, *d_nh = nullptr;
-    HIP_TRY(hipMalloc(&val, total * 8));
-    HIP_TRY(hipMalloc(&LD, total * 8));
-    HIP_TRY(hipMalloc(&SH, total * 8));
-    HIP_TRY(hipMalloc(&NL, total * 4));
-    HIP_TRY(hipMalloc(&hub, (uint64_t)nseg * 4));
-    HIP_TRY(hipMalloc(&d_nh, 8));
+    HIP_TRY(sc.take(&val, total * 8));
+    HIP_TRY(sc.take(&LD, total * 8));
+    HIP_TRY(sc.take(&SH, total * 8));
+    HIP_TRY(sc.take(&NL, total * 4));
+    // a kept plan outlives the build: its own allocations
+    if (keep) HIP_TRY(hipMalloc(&hub, (uint64_t)nseg * 4));
+    else HIP_TRY(sc.take(&hub, (uint64_t)nseg * 4));
+    HIP_TRY(sc.take(&d_nh, 8));
     k_alias_small<W><<<grid_for((uint64_t)nseg * 64, 256, 1u << 16), 256, 0, st>>>(seg, nseg, w, val);
     HIP_TRY(hipGetLastError());
     // segments longer than one chunk
@@ -599,7 +665,7 @@ static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nse
     size_t tmax = tb;
     for (size_t x : {tb2, tb3, tb4}) tmax = x > tmax ? x : tmax;
     void *tmp = nullptr;
-    HIP_TRY(hipMalloc(&tmp, tmax + 16));
+    HIP_TRY(sc.take(&tmp, tmax + 16));
     tb = tmax;
     HIP_TRY(hipcub::DeviceSelect::If(tmp, tb, ids, hub, d_nh, nseg, IsHub{seg}, st));
     uint32_t nh = 0;
@@ -609,18 +675,19 @@ static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nse
     double *csum = nullptr, *hsum = nullptr;
     uint64_t nchunks = 0;
     if (nh) {
-        HIP_TRY(hipMalloc(&cnt, ((uint64_t)nh + 1) * 8));
-        HIP_TRY(hipMalloc(&cstart, ((uint64_t)nh + 1) * 8));
+        HIP_TRY(sc.take(&cnt, ((uint64_t)nh + 1) * 8));
+        if (keep) HIP_TRY(hipMalloc(&cstart, ((uint64_t)nh + 1) * 8));
+        else HIP_TRY(sc.take(&cstart, ((uint64_t)nh + 1) * 8));
         k_alias_hub_chunks<<<grid_for((uint64_t)nh + 1, 256, 4096), 256, 0, st>>>(seg, hub, nh, cnt);
         HIP_TRY(hipGetLastError());
         tb = tmax;
         HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, cstart, (uint64_t)nh + 1, st));
         HIP_TRY(hipMemcpyAsync(&nchunks, cstart + nh, 8, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        HIP_TRY(hipMalloc(&csum, nchunks * 8));
-        HIP_TRY(hipMalloc(&cq, nchunks * 24));
-        HIP_TRY(hipMalloc(&hsum, (uint64_t)nh * 8));
-        HIP_TRY(hipMalloc(&hrem, (uint64_t)nh * 16));
+        HIP_TRY(sc.take(&csum, nchunks * 8));
+        HIP_TRY(sc.take(&cq, nchunks * 24));
+        HIP_TRY(sc.take(&hsum, (uint64_t)nh * 8));
+        HIP_TRY(sc.take(&hrem, (uint64_t)nh * 16));
         const unsigned gc = grid_for(nchunks * 64, 256, 1u << 16), gh = grid_for((uint64_t)nh * 64, 256, 1u << 16);
         k_alias_chunk_sum<W><<<gc, 256, 0, st>>>(seg, hub, cstart, nh, w, csum);
         k_alias_hub_sum<<<gh, 256, 0, st>>>(cstart, nh, csum, hsum);
@@ -642,19 +709,19 @@ static hipError_t alias_segments(felog_ctx *c, const uint64_t *seg, uint32_t nse
                                              total, st));
     k_alias_pair<Out><<<grid_for(total, 256, 1u << 20), 256, 0, st>>>(seg, seg_of, total, val, LD, SH, NL, out);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
-    if (keep && nh) {
-        *keep = HubPlan{hub, cstart, nh, nchunks};
-        hub = nullptr;
-        cstart = nullptr;
+    if (keep) {
+        if (nh) {
+            *keep = HubPlan{hub, cstart, nh, nchunks};
+        } else {
+            dfree(hub);
+            dfree(cstart);
+        }
     }
-    for (void *p : {(void *)val, (void *)LD, (void *)SH, (void *)NL, (void *)hub, (void *)d_nh, tmp, (void *)cnt,
-                    (void *)cstart, (void *)cq, (void *)hrem, (void *)csum, (void *)hsum})
-        dfree(p);
+    sc.release(m0);
     return hipSuccess;
 }
 
-hipError_t build_csr(felog_ctx *c, uint32_t **edge_src) {
+hipError_t build_csr(felog_ctx *c, Scratch &sc, uint32_t **edge_src) {
     const uint64_t E = c->E;
     const uint32_t V = c->V;
     HIP_TRY(hipMalloc(&c->d_row_ptr, (uint64_t)(V + 1) * 8));
@@ -664,12 +731,14 @@ hipError_t build_csr(felog_ctx *c, uint32_t **edge_src) {
     HIP_TRY(hipMalloc(&c->d_w, E * 4 + 4));
 
     // Stable LSD radix sort of (src, dst|w) keeps each vertex's neighbours in
-    // input order.
+    // input order.  The second key buffer sits at the bottom of the scratch:
+    // it may end up holding the sorted sources, which the alias build reads.
     uint64_t *vals_a = nullptr, *vals_b = nullptr;
     uint32_t *keys_b = nullptr;
-    HIP_TRY(hipMalloc(&vals_a, E * 8 + 8));
-    HIP_TRY(hipMalloc(&vals_b, E * 8 + 8));
-    HIP_TRY(hipMalloc(&keys_b, E * 4 + 4));
+    HIP_TRY(sc.take(&keys_b, E * 4 + 4));
+    const size_t m0 = sc.mark();
+    HIP_TRY(sc.take(&vals_a, E * 8 + 8));
+    HIP_TRY(sc.take(&vals_b, E * 8 + 8));
     k_pack<<<grid_for(E, 256, 65535), 256, 0, c->stream>>>(c->d_edst, c->d_ew, E, vals_a);
     HIP_TRY(hipGetLastError());
     int bits = 1;
@@ -679,7 +748,7 @@ hipError_t build_csr(felog_ctx *c, uint32_t **edge_src) {
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, vals, E, 0, bits, c->stream));
     void *d_tmp = nullptr;
-    HIP_TRY(hipMalloc(&d_tmp, tmp + 16));
+    HIP_TRY(sc.take(&d_tmp, tmp + 16));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp, keys, vals, E, 0, bits, c->stream));
     // row_ptr from the sorted sources; degrees and vertex records from it
     k_row_ptr<<<grid_for(E + 1, 256, 65535), 256, 0, c->stream>>>(keys.Current(), E, V, c->d_row_ptr);
@@ -690,28 +759,29 @@ hipError_t build_csr(felog_ctx *c, uint32_t **edge_src) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     // the sorted sources are the vertex of every CSR entry: kept for the
-    // alias build, the other key buffer is released
+    // alias build (c->d_esrc stays set when it holds them)
     *edge_src = keys.Current();
-    dfree(keys.Current() == c->d_esrc ? keys_b : c->d_esrc);
-    dfree(d_tmp);
-    dfree(vals_a);
-    dfree(vals_b);
+    if (keys.Current() != c->d_esrc) {
+        dfree(c->d_esrc);
+        c->d_esrc = nullptr;
+    }
+    sc.release(m0);
     dfree(c->d_edst);
     dfree(c->d_ew);
-    c->d_esrc = c->d_edst = nullptr;
+    c->d_edst = nullptr;
     c->d_ew = nullptr;
     return hipSuccess;
 }
 
 // Edge records (alias columns and both neighbours' walk records, written
 // once per entry by the pairing kernel).
-hipError_t build_alias(felog_ctx *c, const uint32_t *edge_src) {
+hipError_t build_alias(felog_ctx *c, Scratch &sc, const uint32_t *edge_src) {
     const uint64_t E = c->E;
     HIP_TRY(hipMalloc(&c->d_vwalk, (uint64_t)c->V * sizeof(VertexRec) + 8));
     k_vwalk<<<grid_for(c->V, 256, 65535), 256, 0, c->stream>>>(c->d_vrec, c->d_col, c->V, c->d_vwalk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMalloc(&c->d_erec, E * sizeof(EdgeRec) + 16));
-    HIP_TRY(alias_segments(c, c->d_row_ptr, c->V, E, (const float *)c->d_w, edge_src,
+    HIP_TRY(alias_segments(c, sc, c->d_row_ptr, c->V, E, (const float *)c->d_w, edge_src,
                            EdgeOut{c->d_row_ptr, c->d_col, c->d_vwalk, c->d_erec}, &c->hubs));
     return hipSuccess;
 }
@@ -722,7 +792,7 @@ hipError_t build_alias(felog_ctx *c, const uint32_t *edge_src) {
 // totals (src/word2vec.cu:740-747), the negative sampler over deg^0.75
 // (src/word2vec.cu:674-694 draws from the same law through a 1e8 table) and
 // the subsampling thresholds (src/word2vec.cu:180-212), all on the device.
-hipError_t build_vocab_and_tables(felog_ctx *c) {
+hipError_t build_vocab_and_tables(felog_ctx *c, Scratch &sc) {
     const uint32_t V = c->V;
     hipStream_t st = c->stream;
     HIP_TRY(hipMalloc(&c->d_vocab_order, (uint64_t)V * 4));
@@ -730,23 +800,24 @@ hipError_t build_vocab_and_tables(felog_ctx *c) {
     HIP_TRY(hipMalloc(&c->d_in_vocab, (uint64_t)V));
     HIP_TRY(hipMalloc(&c->d_uni, (uint64_t)V * 8));
     HIP_TRY(hipMalloc(&c->d_keep_thr, (uint64_t)V * 4));
+    const size_t m0 = sc.mark();
     uint32_t *keys_a = nullptr, *keys_b = nullptr, *ids_a = nullptr, *ids_b = nullptr;
     uint64_t *totals = nullptr, *seg01 = nullptr;
     double *uw = nullptr;
-    HIP_TRY(hipMalloc(&keys_a, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&keys_b, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&ids_a, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&ids_b, (uint64_t)V * 4));
-    HIP_TRY(hipMalloc(&totals, 16));
-    HIP_TRY(hipMalloc(&seg01, 16));
-    HIP_TRY(hipMalloc(&uw, (uint64_t)V * 8));
+    HIP_TRY(sc.take(&keys_a, (uint64_t)V * 4));
+    HIP_TRY(sc.take(&keys_b, (uint64_t)V * 4));
+    HIP_TRY(sc.take(&ids_a, (uint64_t)V * 4));
+    HIP_TRY(sc.take(&ids_b, (uint64_t)V * 4));
+    HIP_TRY(sc.take(&totals, 16));
+    HIP_TRY(sc.take(&seg01, 16));
+    HIP_TRY(sc.take(&uw, (uint64_t)V * 8));
     k_vocab_keys<<<grid_for(V, 256, 65535), 256, 0, st>>>(c->d_deg, V, keys_a, ids_a, totals, seg01);
     HIP_TRY(hipGetLastError());
     hipcub::DoubleBuffer<uint32_t> keys(keys_a, keys_b), ids(ids_a, ids_b);
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, ids, (uint64_t)V, 0, 32, st));
     void *d_tmp = nullptr;
-    HIP_TRY(hipMalloc(&d_tmp, tmp + 16));
+    HIP_TRY(sc.take(&d_tmp, tmp + 16));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp, keys, ids, (uint64_t)V, 0, 32, st));
     HIP_TRY(hipMemcpyAsync(c->d_vocab_order, ids.Current(), (uint64_t)V * 4, hipMemcpyDeviceToDevice, st));
     k_vocab_rank<<<grid_for(V, 256, 65535), 256, 0, st>>>(c->d_vocab_order, c->d_deg, V, c->opt.min_count,
@@ -758,12 +829,11 @@ hipError_t build_vocab_and_tables(felog_ctx *c) {
     HIP_TRY(hipGetLastError());
     uint64_t h_tot[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(h_tot, totals, 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(alias_segments(c, seg01, 1, V, (const double *)uw, nullptr, PairOut{c->d_uni}, nullptr));
+    HIP_TRY(alias_segments(c, sc, seg01, 1, V, (const double *)uw, nullptr, PairOut{c->d_uni}, nullptr));
+    HIP_TRY(hipStreamSynchronize(st));  // h_tot
     c->vocab_size = h_tot[0];
     c->train_words = h_tot[1];
-    for (void *p : {(void *)keys_a, (void *)keys_b, (void *)ids_a, (void *)ids_b, (void *)totals, (void *)seg01,
-                    (void *)uw, d_tmp})
-        dfree(p);
+    sc.release(m0);
     // n log2 n table for the walk statistics
     const uint32_t L = c->opt.max_walk_len;
     std::vector<double> T(L + 2), dT(L + 1);
@@ -776,7 +846,7 @@ hipError_t build_vocab_and_tables(felog_ctx *c) {
     return hipSuccess;
 }
 
-hipError_t init_tables(felog_ctx *c) {
+hipError_t init_tables(felog_ctx *c, Scratch *sc) {
     const uint64_t n = (uint64_t)c->V * c->dp;
     const size_t es = c->opt.precision == FELOG_BF16 ? 2 : 4;
     // one allocation: syn1neg directly follows syn0, so a row of either
@@ -793,7 +863,9 @@ hipError_t init_tables(felog_ctx *c) {
         a = a * a;
     }
     Affine *d_jump = nullptr;
-    HIP_TRY(hipMalloc(&d_jump, sizeof(jump)));
+    const size_t m0 = sc ? sc->mark() : 0;
+    if (sc) HIP_TRY(sc->take(&d_jump, sizeof(jump)));
+    else HIP_TRY(hipMalloc(&d_jump, sizeof(jump)));
     HIP_TRY(hipMemcpyAsync(d_jump, jump, sizeof(jump), hipMemcpyHostToDevice, c->stream));
     if (es == 4)
         k_init_syn0<float><<<grid_for(n / 8, 256, 1u << 20), 256, 0, c->stream>>>(
@@ -804,7 +876,8 @@ hipError_t init_tables(felog_ctx *c) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(c->d_syn1, 0, n * es, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    dfree(d_jump);
+    if (sc) sc->release(m0);
+    else dfree(d_jump);
     return hipSuccess;
 }
 

@@ -143,13 +143,6 @@ __device__ __forceinline__ uint32_t round_bits(uint32_t &s) {
     return s;
 }
 
-// upper 16 bits of x + r16 (r16 < 2^16): x stochastically rounded to bf16;
-// infinities and NaN are truncated unchanged
-__device__ __forceinline__ uint32_t sr_bits(float x, uint32_t r16) {
-    const uint32_t b = __float_as_uint(x);
-    return (__builtin_isfinite(x) ? b + r16 : b) >> 16;
-}
-
 // bf16 pair += (d0, d1) with stochastic rounding, by compare-and-swap on the
 // containing 32-bit word (no delta lost).
 __device__ __forceinline__ void bf16_pair_add_sr(uint16_t *p, float d0, float d1, uint32_t rnd) {
@@ -158,7 +151,7 @@ __device__ __forceinline__ void bf16_pair_add_sr(uint16_t *p, float d0, float d1
     while (true) {
         const float lo = bf16_to_f32((uint16_t)(old & 0xFFFFu)) + d0;
         const float hi = bf16_to_f32((uint16_t)(old >> 16)) + d1;
-        const uint32_t nw = (uint32_t)f32_to_bf16_sr(lo, rnd) | ((uint32_t)f32_to_bf16_sr(hi, rnd >> 16) << 16);
+        const uint32_t nw = f32x2_to_bf16x2_sr(lo, hi, rnd);
         const uint32_t prev = atomicCAS(w, old, nw);
         if (prev == old) break;
         old = prev;
@@ -208,10 +201,10 @@ __device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, 
         if constexpr (WB) {
             const uint2 old = *(const uint2 *)(staged + col0);
             uint2 nw;
-            nw.x = sr_bits(__uint_as_float(old.x << 16) + acc[0], b0 & 0xFFFFu) |
-                   (sr_bits(__uint_as_float(old.x & 0xFFFF0000u) + acc[1], b0 >> 16) << 16);
-            nw.y = sr_bits(__uint_as_float(old.y << 16) + acc[2], b1 & 0xFFFFu) |
-                   (sr_bits(__uint_as_float(old.y & 0xFFFF0000u) + acc[3], b1 >> 16) << 16);
+            nw.x = f32x2_to_bf16x2_sr(__uint_as_float(old.x << 16) + acc[0],
+                                      __uint_as_float(old.x & 0xFFFF0000u) + acc[1], b0);
+            nw.y = f32x2_to_bf16x2_sr(__uint_as_float(old.y << 16) + acc[2],
+                                      __uint_as_float(old.y & 0xFFFF0000u) + acc[3], b1);
             *(uint2 *)dst = nw;
         } else {
             bf16_pair_add_sr(dst, acc[0], acc[1], b0);
@@ -396,8 +389,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         nxt2 = nxt < n_tiles ? next_live() : n_tiles;
         if constexpr (sizeof(T) == 2) {
             // a tile's rounding bits depend on (round, slot, thread) only,
-            // not on the workgroup that takes it
-            rs = draw(a.seed, tid, DOM_ROUND_BF16, (uint32_t)tile, round).x;
+            // not on the workgroup that takes it: one draw per tile (scalar
+            // unit), hashed with the thread index
+            const uint32_t ts = __builtin_amdgcn_readfirstlane((uint32_t)tile);
+            const uint32_t base = draw(a.seed, 0, DOM_ROUND_BF16, ts, round).x;
+            rs = mix32(base + tid * 0x9E3779B9u);
             if (rs == 0) rs = 0x9E3779B9u;
         }
         for (uint32_t i = tid; i < ((OMAX + L.wrows) * GT + OMAX) / 4; i += kBlock)
@@ -539,13 +535,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 } else {
                     // G = hi + lo in bf16; each 4-row block of G^T holds its
                     // four hi values, then its four lo values
-                    uint32_t h[4], l[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        h[j] = f32_to_bf16_rne(gv[j]);
-                        l[j] = f32_to_bf16_rne(gv[j] - bf16_to_f32((uint16_t)h[j]));
-                    }
-                    *(uint4 *)cp = make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, l[0] | l[1] << 16, l[2] | l[3] << 16);
+                    const uint32_t h01 = f32x2_to_bf16x2_rne(gv[0], gv[1]);
+                    const uint32_t h23 = f32x2_to_bf16x2_rne(gv[2], gv[3]);
+                    const uint32_t l01 = f32x2_to_bf16x2_rne(gv[0] - __uint_as_float(h01 << 16),
+                                                             gv[1] - __uint_as_float(h01 & 0xFFFF0000u));
+                    const uint32_t l23 = f32x2_to_bf16x2_rne(gv[2] - __uint_as_float(h23 << 16),
+                                                             gv[3] - __uint_as_float(h23 & 0xFFFF0000u));
+                    *(uint4 *)cp = make_uint4(h01, h23, l01, l23);
                 }
                 if (dbg_on)
                     for (int j = 0; j < 4; ++j) a.dbg_logits[(tile * RMAX + r0 + j) * OMAX + o] = acc[j];
