@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# FELOG_ORACLE_LIB selects another build of the same sources (the
+# address/undefined-behaviour sanitizer build, liboracle_san.so)
+LIB_PATH = os.environ.get("FELOG_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _lib = None
 
