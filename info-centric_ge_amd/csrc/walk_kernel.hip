@@ -18,7 +18,8 @@
 //     degree-2 vertex whose first neighbour is itself ends it (:977-979), a
 //     self loop drawn at step 0 is redrawn (:981-983, at most 64 times).
 // Walk i of the round writes its path into a fixed slot of `slot` vertices
-// (walker-major), continuing in an overflow pool if it grows past the slot.
+// (walker-major), continuing in an overflow pool if it grows past the slot;
+// the first 32 vertices are written when the walk ends, from LDS.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include "felog_dev.h"
@@ -54,6 +55,32 @@ constexpr uint64_t kNoOverflow = ~0ull;
 
 __device__ __forceinline__ uint32_t path_at(const WalkArgs &a, uint32_t w, uint64_t ob, uint32_t k) {
     return k < a.slot ? a.slots[(uint64_t)w * a.slot + k] : a.ovf[ob + (k - a.slot)];
+}
+
+// Writes the first min(len, kHistLds) vertices of a finished walk from the
+// lane's LDS history to its slot (16-byte stores when the slot is a multiple
+// of four vertices) and, past the slot, to its overflow run.  A lane writes
+// its whole prefix at once, so the slot's lines are filled in one burst
+// instead of one 4-byte store per step into a different line each time.
+__device__ __forceinline__ void flush_path(const WalkArgs &a, const uint32_t *H, uint32_t w, uint64_t ob,
+                                           uint32_t len) {
+    const uint32_t nh = len < (uint32_t)kHistLds ? len : (uint32_t)kHistLds;
+    const uint32_t ns = nh < a.slot ? nh : a.slot;
+    uint32_t *dst = a.slots + (uint64_t)w * a.slot;
+    if ((a.slot & 3u) == 0) {
+        // k + 3 < slot (slot a multiple of 4) and k + 3 < kHistLds (k <= 28)
+        for (uint32_t k = 0; k < ns; k += 4) {
+            uint4 q;
+            q.x = H[k * 64];
+            q.y = H[(k + 1) * 64];
+            q.z = H[(k + 2) * 64];
+            q.w = H[(k + 3) * 64];
+            *reinterpret_cast<uint4 *>(dst + k) = q;
+        }
+    } else {
+        for (uint32_t k = 0; k < ns; ++k) dst[k] = H[k * 64];
+    }
+    for (uint32_t k = a.slot; k < nh; ++k) a.ovf[ob + (k - a.slot)] = H[k * 64];
 }
 
 __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
@@ -94,7 +121,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                     ob = kNoOverflow;
                     walk_stats_init(st);
                     H[0] = v;
-                    a.slots[(uint64_t)w * a.slot] = v;
                     active = true;
                 }
             }
@@ -129,21 +155,24 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
                             for (uint32_t k = kHistLds; k <= s; ++k) f += (path_at(a, w, ob, k) == y) ? 1u : 0u;
                         }
                         const uint32_t ns = s + 1;
-                        if (ns < a.slot) {
+                        if (ns >= a.slot && ob == kNoOverflow) {
+                            const uint64_t need_n = a.max_len - a.slot;
+                            const uint64_t got = atomicAdd(&a.counters[C_OVF_NEXT], (unsigned long long)need_n);
+                            if (got + need_n > a.ovf_cap) {
+                                atomicAdd(&a.counters[C_OVF_FAIL], 1ull);
+                                return true;  // truncated; the host reports FELOG_ERR_CAPACITY
+                            }
+                            ob = got;
+                        }
+                        // the first kHistLds vertices stay in LDS until the
+                        // walk ends (flush_path); later ones go straight out
+                        if (ns < (uint32_t)kHistLds) {
+                            H[ns * 64] = y;
+                        } else if (ns < a.slot) {
                             a.slots[(uint64_t)w * a.slot + ns] = y;
                         } else {
-                            if (ob == kNoOverflow) {
-                                const uint64_t need_n = a.max_len - a.slot;
-                                const uint64_t got = atomicAdd(&a.counters[C_OVF_NEXT], (unsigned long long)need_n);
-                                if (got + need_n > a.ovf_cap) {
-                                    atomicAdd(&a.counters[C_OVF_FAIL], 1ull);
-                                    return true;  // truncated; the host reports FELOG_ERR_CAPACITY
-                                }
-                                ob = got;
-                            }
                             a.ovf[ob + (ns - a.slot)] = y;
                         }
-                        if (ns < (uint32_t)kHistLds) H[ns * 64] = y;
                         s = ns;
                         return lim ? s + 1 >= lim : walk_stats_step(st, a.T, a.dT, f, s + 1, s, a.min_L);
                     };
@@ -163,6 +192,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
             }
         }
         if (stop) {
+            flush_path(a, H, w, ob, s + 1);
             a.lens[w] = s + 1;
             a.ovf_base[w] = ob;
             steps += s;
