@@ -151,7 +151,7 @@ def end_to_end(src, dst, w, n, args, model):
         f.close()
         f = felog.Felog(**opts)
         f.set_edges(ts, td, tw, n, make_undirected=True)  # host edge list to the device: input I/O, not timed
-        f.set_tile_shape(shape["rows"], shape["waves_per_simd"], shape["groups_per_cu"])
+        f.set_tile_shape(*(shape[k] for k in felog.SHAPE_FIELDS))
         extra = {}
         # timed: CSR + alias + vocabulary + tables, the budget's prediction,
         # every round to the stop
@@ -181,7 +181,7 @@ def end_to_end(src, dst, w, n, args, model):
                          rounds=st["rounds_done"], tokens=vs["visits"], tokens_per_s=vs["visits"] / (t2 - t0),
                          auc=auc, visit_kl=vs["kl"], visit_entropy=vs["entropy"],
                          rounds_gpu_ms=st["graph_ms_total"],
-                         tile_shape={k: shape[k] for k in ("rows", "waves_per_simd", "groups_per_cu")}, **extra)
+                         tile_shape={k: shape[k] for k in felog.SHAPE_FIELDS}, **extra)
         f.close()
     p, b = out["plain_fp32"], out["bf16_surrogate"]
     out["throughput_ratio"] = p["wall_ms"] / b["wall_ms"]
@@ -384,12 +384,11 @@ def main():
                           "flop_per_pair": 6 * 6 * args.dim},
         "roofline": roof,
         "tile_autotune": {
-            "chosen": {k: chosen[k] for k in ("rows", "waves_per_simd", "groups_per_cu")},
-            "default": {k: default_shape[k] for k in ("rows", "waves_per_simd", "groups_per_cu")},
+            "chosen": {k: chosen[k] for k in felog.SHAPE_FIELDS},
+            "default": {k: default_shape[k] for k in felog.SHAPE_FIELDS},
             "seconds": tune_s,
-            "shapes": [[t["rows"], t["waves_per_simd"], t["groups_per_cu"], round(t["tile_ms"], 3)]
-                       for t in tuned],
-            "fields": "rows, waves_per_simd, groups_per_cu, tile kernel ms on the warm-up corpus"},
+            "shapes": [[t[k] for k in felog.SHAPE_FIELDS] + [round(t["tile_ms"], 3)] for t in tuned],
+            "fields": ", ".join(felog.SHAPE_FIELDS) + ", tile kernel ms on the warm-up corpus"},
         "cpu_baseline": None,
     }
     f.close()

@@ -236,14 +236,24 @@ typedef struct felog_tile_shape {
     uint32_t groups_per_cu;   /* persistent workgroups per CU (at most what
                                  LDS and registers allow; with rows 32, of
                                  the 32-row kernel)                         */
+    uint32_t windows_per_wave; /* centre windows whose pairs one wavefront
+                                 counts at a time: 1, 2, 4 or 8 (64, 32, 16
+                                 or 8 lanes per window; default 8)          */
+    uint32_t dim_split;       /* columns of a 16-row block that one wavefront
+                                 updates in the dVin / dUout products: dp,
+                                 dp/2 or dp/4 (dp = d rounded up to 32; d =
+                                 128: 128, 64 or 32; default dp/4)          */
     float tile_ms;            /* felog_tune_sgns: tile-kernel time          */
 } felog_tile_shape;
 felog_status felog_tile_shape_set(felog_ctx *ctx, const felog_tile_shape *shape);
 felog_status felog_tile_shape_get(felog_ctx *ctx, felog_tile_shape *shape);
-/* Times every tile shape on the first sample_sentences sentences of the
- * current walk corpus (reps timed passes each, after one untimed), with the
- * tables restored after every pass, and installs the fastest.  results
- * (up to max_results) receive every shape tried with its mean time. */
+/* Times tile shapes on the first sample_sentences sentences of the current
+ * walk corpus (reps timed passes each, after one untimed), with the tables
+ * restored after every pass, and installs the fastest: first every rows x
+ * waves_per_simd x groups_per_cu (default windows and split), then every
+ * windows_per_wave {1, 2, 4, 8} x dim_split {dp/4, dp/2, dp} at the best
+ * of those.  results (up to max_results) receive every shape tried with
+ * its mean time. */
 felog_status felog_tune_sgns(felog_ctx *ctx, uint64_t sample_sentences,
                              uint32_t reps, felog_tile_shape *results,
                              uint32_t max_results, uint32_t *n_results);

@@ -77,64 +77,93 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t grp = lane >> 4, l = lane & 15;
     unsigned long long n_eval = 0, n_conv = 0;
-    for (uint32_t v = wave; v < V; v += nwaves) {
-        // vertices with a learned walk budget take no convergence test,
-        // except the recheck's when their count ends this round
-        if (!in_vocab[v] || flags[v]) continue;
-        const uint32_t bc = budget_cnt ? budget_cnt[v] : 0u;
-        if (recheck ? bc != round : bc != 0u) continue;
-        const uint64_t b = row_ptr[v];
-        const uint32_t deg = (uint32_t)(row_ptr[v + 1] - b);
+    // Each wavefront takes 64 consecutive vertices at a time: every lane
+    // tests one for the evaluation (in vocabulary, not converged, budget
+    // state) and reads its CSR range, then the wavefront scores the chosen
+    // ones one after the other.
+    for (uint32_t base = wave * 64; base < V; base += nwaves * 64) {
+        const uint32_t vl = base + lane;
+        bool pick = false;
+        uint64_t lb = 0, le = 0;
+        if (vl < V && in_vocab[vl] && !flags[vl]) {
+            // vertices with a learned walk budget take no convergence test,
+            // except the recheck's when their count ends this round
+            const uint32_t bc = budget_cnt ? budget_cnt[vl] : 0u;
+            pick = recheck ? bc == round : bc == 0u;
+            if (pick) {
+                lb = row_ptr[vl];
+                le = row_ptr[vl + 1];
+            }
+        }
+        for (uint64_t m = __ballot(pick); m; m &= m - 1) {
+        const uint32_t src = (uint32_t)__builtin_ctzll(m);
+        const uint32_t v = base + src;
+        const uint64_t b = __shfl(lb, (int)src);
+        const uint32_t deg = (uint32_t)(__shfl(le, (int)src) - b);
         float score = 0.f;
         if (deg > 0) {
             const uint32_t cnt = deg < kEvalNbrs ? deg : kEvalNbrs;
             // chosen neighbour index held by lane i (i < cnt)
             uint32_t mine = lane;
             if (deg > kEvalNbrs) {
+                // Floyd's subset: draw i (lane i, all at once) picks
+                // t_i in [0, j_i], j_i = deg - 30 + i; taken already -> j_i
+                const uint32_t jl = deg - kEvalNbrs + lane;
+                uint32_t t = 0;
+                if (lane < kEvalNbrs) t = draw(seed, v, DOM_EVAL | lane, 0xFFFFFFFFu, round).x % (jl + 1);
                 mine = 0xFFFFFFFFu;
                 for (uint32_t i = 0; i < kEvalNbrs; ++i) {
-                    const uint32_t j = deg - kEvalNbrs + i;
-                    const uint32_t t = draw(seed, v, DOM_EVAL | i, 0xFFFFFFFFu, round).x % (j + 1);
-                    const bool dup = __ballot(lane < i && mine == t) != 0;
-                    if (lane == i) mine = dup ? j : t;
+                    const uint32_t ti = __shfl(t, (int)i);
+                    const bool dup = __ballot(lane < i && mine == ti) != 0;
+                    if (lane == i) mine = dup ? jl : t;
                 }
             }
+            // neighbour vertex of lane i (i < cnt), one gather for all
+            const uint32_t nbr = lane < cnt ? col[b + mine] : v;
             // Four neighbours at a time, one per 16-lane group: lane l of a
             // group holds columns 8l .. 8l+7 of every 128-column block and
             // the group reduces the dot product and the norm over 16 lanes.
-            const uint32_t grp = lane >> 4, l = lane & 15;
+            // All eight rounds of four neighbours are loaded before any is
+            // reduced.
             const T *rv = syn0 + (uint64_t)v * dp;
-            float nn = 0.f;
+            const T *ru[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t i = 4 * k + grp;
+                ru[k] = syn0 + (uint64_t)__shfl(nbr, (int)(i & 63)) * dp;
+            }
+            float nn = 0.f, dot[8], mm[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) dot[k] = mm[k] = 0.f;
             for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
-                float x[8];
+                float x[8], y[8][8];
                 load8(rv, e0, d, x);
 #pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (4 * k < cnt) load8(ru[k], e0, d, y[k]);
+#pragma unroll
                 for (int j = 0; j < 8; ++j) nn += x[j] * x[j];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (4 * k >= cnt) continue;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        dot[k] += x[j] * y[k][j];
+                        mm[k] += y[k][j] * y[k][j];
+                    }
+                }
             }
             nn = group_sum16(nn);
             const float nv = sqrtf(nn);
             float sum = 0.f;  // this group's neighbours
-            for (uint32_t i0 = 0; i0 < cnt; i0 += 4) {
-                const uint32_t i = i0 + grp;
-                const uint32_t idx = __shfl(mine, (int)(i & 63));
-                const bool have = i < cnt;
-                const T *ru = syn0 + (uint64_t)(have ? col[b + idx] : v) * dp;
-                float dot = 0.f, mm = 0.f;
-                for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
-                    float x[8], y[8];
-                    load8(rv, e0, d, x);
-                    load8(ru, e0, d, y);
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        dot += x[j] * y[j];
-                        mm += y[j] * y[j];
-                    }
-                }
-                dot = group_sum16(dot);
-                mm = group_sum16(mm);
-                const float nu = sqrtf(mm);
-                if (have) sum += (nv == 0.f || nu == 0.f) ? 0.f : dot / (nv * nu);
+            for (int k = 0; k < 8; ++k) {
+                if (4 * k >= cnt) break;
+                const float dk = group_sum16(dot[k]);
+                const float nu = sqrtf(group_sum16(mm[k]));
+                if (4 * k + grp < cnt) sum += (nv == 0.f || nu == 0.f) ? 0.f : dk / (nv * nu);
             }
             sum += __shfl_xor(sum, 16);
             sum += __shfl_xor(sum, 32);
@@ -152,6 +181,7 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                     ++n_eval;
                 }
             }
+        }
         }
     }
     if (lane == 0) {

@@ -847,14 +847,30 @@ felog_status check_shape(felog_ctx *c, const felog_tile_shape &s) {
         return fail(c, FELOG_ERR_ARG,
                     "waves_per_simd must be " + std::to_string(lo) + " or " + std::to_string(lo + 1));
     if (s.groups_per_cu > 64) return fail(c, FELOG_ERR_ARG, "groups_per_cu must be <= 64");
+    if (s.windows_per_wave != 0 && s.windows_per_wave != 1 && s.windows_per_wave != 2 && s.windows_per_wave != 4 &&
+        s.windows_per_wave != 8)
+        return fail(c, FELOG_ERR_ARG, "windows_per_wave must be 1, 2, 4 or 8");
+    if (s.dim_split != 0 && s.dim_split != c->dp && 2 * s.dim_split != c->dp && 4 * s.dim_split != c->dp)
+        return fail(c, FELOG_ERR_ARG, "dim_split must be " + std::to_string(c->dp) + ", " +
+                                          std::to_string(c->dp / 2) + " or " + std::to_string(c->dp / 4) +
+                                          " (d rounded up to 32, divided by 1, 2 or 4)");
     return FELOG_OK;
 }
 
-void apply_shape(felog_ctx *c, uint32_t rows, uint32_t wpe, uint32_t gpc) {
-    if (rows != c->tile_rows || wpe != c->tile_wpe || gpc != c->tile_gpc) drop_round_graph(c);
+// column parts of a dim split (0: the default)
+uint32_t split_parts(const felog_ctx *c, uint32_t dim_split) {
+    return dim_split ? c->dp / dim_split : 0u;
+}
+
+void apply_shape(felog_ctx *c, uint32_t rows, uint32_t wpe, uint32_t gpc, uint32_t wpw, uint32_t parts) {
+    if (rows != c->tile_rows || wpe != c->tile_wpe || gpc != c->tile_gpc || wpw != c->tile_wpw ||
+        parts != c->tile_parts)
+        drop_round_graph(c);
     c->tile_rows = rows;
     c->tile_wpe = wpe;
     c->tile_gpc = gpc;
+    c->tile_wpw = wpw;
+    c->tile_parts = parts;
 }
 
 }  // namespace
@@ -867,7 +883,7 @@ felog_status felog_tile_shape_set(felog_ctx *c, const felog_tile_shape *shape) {
     const felog_tile_shape &s = shape ? *shape : zero;
     felog_status st = check_shape(c, s);
     if (st) return st;
-    apply_shape(c, s.rows, s.waves_per_simd, s.groups_per_cu);
+    apply_shape(c, s.rows, s.waves_per_simd, s.groups_per_cu, s.windows_per_wave, split_parts(c, s.dim_split));
     return FELOG_OK;
 }
 
@@ -879,6 +895,8 @@ felog_status felog_tile_shape_get(felog_ctx *c, felog_tile_shape *out) {
     int per_cu = 0;
     TRY(c, tile_occupancy(c, out->rows, out->waves_per_simd, &per_cu));
     out->groups_per_cu = c->tile_gpc && (int)c->tile_gpc < per_cu ? c->tile_gpc : (uint32_t)per_cu;
+    out->windows_per_wave = c->tile_wpw ? c->tile_wpw : 8u;
+    out->dim_split = c->dp / (c->tile_parts ? c->tile_parts : 4u);
     out->tile_ms = 0.f;
     return FELOG_OK;
 }
@@ -904,26 +922,50 @@ felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t r
         for (uint32_t wpe : {lo, lo + 1}) {
             int per_cu = 0;
             TRY(c, tile_occupancy(c, rows, wpe, &per_cu));
-            for (int g = 1; g <= per_cu; ++g) cand.push_back(felog_tile_shape{rows, wpe, (uint32_t)g, 0.f});
+            for (int g = 1; g <= per_cu; ++g)
+                cand.push_back(felog_tile_shape{rows, wpe, (uint32_t)g, 0u, 0u, 0.f});
         }
     }
     // both tables (one allocation) are restored before every pass
     const size_t tbytes = 2 * (size_t)c->V * c->dp * (c->opt.precision == FELOG_BF16 ? 2 : 4);
     void *snap = nullptr;
     TRY(c, hipMalloc(&snap, tbytes));
-    const uint32_t keep_rows = c->tile_rows, keep_wpe = c->tile_wpe, keep_gpc = c->tile_gpc;
+    const uint32_t keep_rows = c->tile_rows, keep_wpe = c->tile_wpe, keep_gpc = c->tile_gpc,
+                   keep_wpw = c->tile_wpw, keep_parts = c->tile_parts;
     hipError_t e = hipMemcpyAsync(snap, c->d_syn0, tbytes, hipMemcpyDeviceToDevice, c->stream);
-    for (size_t i = 0; i < cand.size() && e == hipSuccess; ++i) {
-        apply_shape(c, cand[i].rows, cand[i].waves_per_simd, cand[i].groups_per_cu);
-        double sum = 0.0;
-        for (uint32_t r = 0; r <= reps && e == hipSuccess; ++r) {
-            e = hipMemcpyAsync(c->d_syn0, snap, tbytes, hipMemcpyDeviceToDevice, c->stream);
-            if (e == hipSuccess) e = round_set(c, c->corpus.round, -1.f, (uint32_t)n);
-            float ms[3] = {0.f, 0.f, 0.f};
-            if (e == hipSuccess) e = run_sgns(c, c->d_off, c->d_tok, n, nt, nullptr, ms);
-            if (r > 0) sum += ms[2];
+    auto time_shapes = [&](size_t from) {
+        for (size_t i = from; i < cand.size() && e == hipSuccess; ++i) {
+            apply_shape(c, cand[i].rows, cand[i].waves_per_simd, cand[i].groups_per_cu, cand[i].windows_per_wave,
+                        split_parts(c, cand[i].dim_split));
+            double sum = 0.0;
+            for (uint32_t r = 0; r <= reps && e == hipSuccess; ++r) {
+                e = hipMemcpyAsync(c->d_syn0, snap, tbytes, hipMemcpyDeviceToDevice, c->stream);
+                if (e == hipSuccess) e = round_set(c, c->corpus.round, -1.f, (uint32_t)n);
+                float ms[3] = {0.f, 0.f, 0.f};
+                if (e == hipSuccess) e = run_sgns(c, c->d_off, c->d_tok, n, nt, nullptr, ms);
+                if (r > 0) sum += ms[2];
+            }
+            cand[i].tile_ms = (float)(sum / reps);
         }
-        cand[i].tile_ms = (float)(sum / reps);
+    };
+    auto fastest = [&]() {
+        size_t best = 0;
+        for (size_t i = 1; i < cand.size(); ++i)
+            if (cand[i].tile_ms < cand[best].tile_ms) best = i;
+        return best;
+    };
+    // rows x register budget x workgroups per CU, then windows per
+    // wavefront x dim split at the fastest of those
+    time_shapes(0);
+    const size_t first = cand.size();
+    if (e == hipSuccess) {
+        const felog_tile_shape b = cand[fastest()];
+        for (uint32_t wpw : {1u, 2u, 4u, 8u})
+            for (uint32_t parts : {4u, 2u, 1u})
+                if (!(wpw == 8u && parts == 4u))  // the default, timed above
+                    cand.push_back(felog_tile_shape{b.rows, b.waves_per_simd, b.groups_per_cu, wpw, c->dp / parts,
+                                                    0.f});
+        time_shapes(first);
     }
     const hipError_t e2 = hipMemcpyAsync(c->d_syn0, snap, tbytes, hipMemcpyDeviceToDevice, c->stream);
     const hipError_t e3 = hipStreamSynchronize(c->stream);
@@ -931,13 +973,17 @@ felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t r
     if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = e3;
     if (e != hipSuccess) {
-        apply_shape(c, keep_rows, keep_wpe, keep_gpc);
+        apply_shape(c, keep_rows, keep_wpe, keep_gpc, keep_wpw, keep_parts);
         return hip_fail(c, e, "felog_tune_sgns");
     }
-    size_t best = 0;
-    for (size_t i = 1; i < cand.size(); ++i)
-        if (cand[i].tile_ms < cand[best].tile_ms) best = i;
-    apply_shape(c, cand[best].rows, cand[best].waves_per_simd, cand[best].groups_per_cu);
+    // shapes of the first sweep report the defaults they ran with
+    for (size_t i = 0; i < first; ++i) {
+        cand[i].windows_per_wave = 8u;
+        cand[i].dim_split = c->dp / 4u;
+    }
+    const size_t best = fastest();
+    apply_shape(c, cand[best].rows, cand[best].waves_per_simd, cand[best].groups_per_cu,
+                cand[best].windows_per_wave, split_parts(c, cand[best].dim_split));
     const uint32_t m = (uint32_t)(cand.size() < max_results ? cand.size() : max_results);
     if (results)
         for (uint32_t i = 0; i < m; ++i) results[i] = cand[i];

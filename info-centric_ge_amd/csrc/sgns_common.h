@@ -63,6 +63,11 @@ struct Params {
     // tiles of this launch: n_in in [nin_lo, nin_hi] (a 32-row kernel takes
     // the small tiles, a 48/64-row one the rest)
     uint32_t nin_lo, nin_hi;
+    // tile shape: log2 of the lanes that count one centre's window (3: eight
+    // windows per wavefront), and the column parts of a row block in the
+    // update products (a wavefront's item is one row block's columns
+    // part, part + parts, ...: dp / parts columns)
+    uint32_t win_shift, col_parts;
     unsigned long long *counters;
     // test hook: when dbg_max > 0 the tables are not touched
     uint32_t dbg_max;

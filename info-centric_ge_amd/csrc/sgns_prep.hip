@@ -309,6 +309,12 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
     a.bgroup = sgns_group_centres(c->opt.batch_size, c->opt.window, c->opt.negative);
     // tiles are packed for 48 or 64 rows (a 32-row shape splits them by size)
     a.rmax = c->tile_rows >= 48 ? c->tile_rows : sgns_tile_rows(a.bgroup, c->opt.window);
+    {
+        // windows per wavefront 1, 2, 4 or 8: 64, 32, 16 or 8 lanes per window
+        const uint32_t wpw = c->tile_wpw ? c->tile_wpw : 8u;
+        a.win_shift = wpw >= 8 ? 3u : wpw >= 4 ? 4u : wpw >= 2 ? 5u : 6u;
+        a.col_parts = c->tile_parts ? c->tile_parts : 4u;
+    }
     a.compat_break = c->opt.compat_vocab_break;
     a.writeback = c->opt.update_mode != FELOG_UPDATE_ATOMIC;
     a.seed = c->opt.seed;

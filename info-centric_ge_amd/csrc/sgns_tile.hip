@@ -179,15 +179,15 @@ __device__ __forceinline__ LaneRow<T> lane_row(T *table, const uint32_t *ids, ui
 
 // Write the lane's four values (columns col0..col0+3 of its row; padding
 // columns past d hold zero deltas on zero rows, so whole vectors are
-// written).  f32 write-back: acc already holds the staged value plus the
-// delta; f32 atomic: acc is the delta.  bf16 write-back: staged values plus
-// deltas, stochastically rounded, one 8-byte store; bf16 atomic: two
-// compare-and-swap pairs.
+// written).  Write-back: acc already holds the staged value plus the delta
+// (the products accumulate onto the staged row); f32 stores it, bf16 rounds
+// it stochastically (16 random bits per element) and stores 8 bytes.
+// Atomic: acc is the delta; f32 adds it, bf16 by two compare-and-swap pairs.
+// The rounding stream advances on every lane, live row or not.
 template <typename T, bool WB>
-__device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, uint32_t col0, const f4 &acc,
-                                          uint32_t &rs) {
-    if (!R.ok) return;
+__device__ __forceinline__ void store_row(const LaneRow<T> &R, uint32_t col0, const f4 &acc, uint32_t &rs) {
     if constexpr (sizeof(T) == 4) {
+        if (!R.ok) return;
         float *dst = (float *)R.p + col0;
         if constexpr (WB) {
             *(f4 *)dst = acc;
@@ -199,18 +199,24 @@ __device__ __forceinline__ void store_row(const LaneRow<T> &R, const T *staged, 
         uint16_t *dst = (uint16_t *)R.p + col0;
         const uint32_t b0 = round_bits(rs), b1 = round_bits(rs);
         if constexpr (WB) {
-            const uint2 old = *(const uint2 *)(staged + col0);
             uint2 nw;
-            nw.x = f32x2_to_bf16x2_sr(__uint_as_float(old.x << 16) + acc[0],
-                                      __uint_as_float(old.x & 0xFFFF0000u) + acc[1], b0);
-            nw.y = f32x2_to_bf16x2_sr(__uint_as_float(old.y << 16) + acc[2],
-                                      __uint_as_float(old.y & 0xFFFF0000u) + acc[3], b1);
-            *(uint2 *)dst = nw;
+            nw.x = f32x2_to_bf16x2_sr(acc[0], acc[1], b0);
+            nw.y = f32x2_to_bf16x2_sr(acc[2], acc[3], b1);
+            if (R.ok) *(uint2 *)dst = nw;
         } else {
+            if (!R.ok) return;
             bf16_pair_add_sr(dst, acc[0], acc[1], b0);
             bf16_pair_add_sr(dst + 2, acc[2], acc[3], b1);
         }
     }
+}
+
+// The staged bf16 values of columns col0..col0+3 (the accumulator's start
+// for a write-back update)
+__device__ __forceinline__ f4 staged_bf16(const uint16_t *staged, uint32_t col0) {
+    const uint2 old = *(const uint2 *)(staged + col0);
+    return f4{__uint_as_float(old.x << 16), __uint_as_float(old.x & 0xFFFF0000u), __uint_as_float(old.y << 16),
+              __uint_as_float(old.y & 0xFFFF0000u)};
 }
 
 // WPE: waves per SIMD the register budget is cut for (the LDS of one
@@ -250,13 +256,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const uint32_t vpr = dp / EPV;
     const uint32_t st_r0 = tid / vpr, st_c0 = tid % vpr;
     const uint32_t st_dr = kBlock / vpr, st_dc = kBlock % vpr;
-    // pair counts: eight threads per centre, strided over its window
-    const uint32_t ci = tid >> 3, cs = tid & 7, span = 2 * w + 1;
+    // pair counts: 2^win_shift lanes per centre (64 >> win_shift windows
+    // per wavefront), strided over its window
+    const uint32_t wsh = a.win_shift, lpw = 1u << wsh, cs = tid & (lpw - 1), span = 2 * w + 1;
+    // update products: items (row block, column part) dealt round-robin to
+    // the four wavefronts, input row blocks first, then output row blocks
+    const uint32_t parts = a.col_parts;
     // negative slot q of a tile belongs to its group q / k (q < 32: the
     // rounded float quotient is exact)
     const float inv_k = k ? 1.f / (float)k : 0.f;
-    // f32 write-back tiles start from the staged rows (acc = old + delta)
-    constexpr bool acc_old = WB && sizeof(T) == 4 && !DBG;
+    // write-back tiles start the products from the staged rows (acc = old +
+    // delta); the test hook takes the deltas alone
+    constexpr bool acc_old = WB && !DBG;
     // stochastic-rounding stream of this thread (bf16 tables), seeded per tile
     uint32_t rs = 0;
 
@@ -413,14 +424,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         // Per pair: P += 1 and the skipped negatives subtracted at [ro][r]
         // (one atomic), W += 1 (one atomic); Q is completed from W and the
         // slot masks when G is formed.
-        if (ci < nc) {
+        for (uint32_t ci = tid >> wsh; ci < nc; ci += kBlock >> wsh) {
             const uint32_t b = td.bshrink[ci];
             const uint32_t ro = td.cen_rout[ci];
             const uint32_t grp = (c0 + ci) / B - g0;  // this centre's group within the tile
             const uint32_t coll = td.cen_coll[ci];     // its group's negatives equal to it
             const uint32_t pinc = 1u - (coll << 16);
             uint32_t *wrow = wcnt + grp * GT;
-            for (uint32_t aa = cs; aa < span; aa += 8) {
+            for (uint32_t aa = cs; aa < span; aa += lpw) {
                 if (aa == w || aa < b || aa >= span - b) continue;
                 const int p = (int)(c0 + ci) - (int)w + (int)aa - (int)lo;  // input position in the tile
                 if (p < 0 || p >= (int)npos) continue;
@@ -557,19 +568,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             const uint32_t grp = lane >> 4, jq = l16 >> 2, qq = l16 & 3;
             const uint32_t gb = (uint32_t)L.o_g, rb = (uint32_t)L.o_rows, rowb = ldv * 2;
             // dVin = G * Uout, K = all OMAX output rows (zero G past n_out)
-            for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
+            const uint32_t ndv = (R16 >> 4) * parts;
+            for (uint32_t it = wave; it < ndv; it += 4) {
+                const uint32_t rt = it / parts, part = it - rt * parts;
                 const uint32_t r = rt * 16 + l16;
                 const LaneRow<T> row = lane_row(syn0, td.rin_tok, r, n_in, dp);
                 // G[r'][o], r' = 16*rt + lane%16, o = 8*grp + s: transposed from G^T
                 const uint32_t a0 = gb + (8 * grp + jq) * GT * 4 + (rt * 4 + qq) * 16;
                 const bf8 ghi = join8(tr_load(l3, a0), tr_load(l3, a0 + 16 * GT));
                 const bf8 glo = join8(tr_load(l3, a0 + 8), tr_load(l3, a0 + 16 * GT + 8));
-                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                for (uint32_t dt = part; dt < dtiles; dt += parts) {
                     const uint32_t col0 = dt * 16 + g4;
                     // Uout[o][16*dt + lane%16], transposed from the staged rows
                     const uint32_t b0 = rb + (RM + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
                     const bf8 uv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
-                    f4 acc = {0.f, 0.f, 0.f, 0.f};
+                    f4 acc = acc_old ? staged_bf16((const uint16_t *)Vs + r * ldv, col0) : f4{0.f, 0.f, 0.f, 0.f};
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, ghi, acc, 0, 0, 0);
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, glo, acc, 0, 0, 0);
                     if constexpr (DBG) {
@@ -577,19 +590,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
                     } else {
-                        store_row<T, WB>(row, Vs + r * ldv, col0, acc, rs);
+                        store_row<T, WB>(row, col0, acc, rs);
                     }
                 }
             }
             // dUout = G^T * Vin, K = R16 rounded up to 32 (zero G past n_in;
             // input rows past RM are output rows, finite)
             const uint32_t nk32 = (R16 + 31) >> 5;
-            for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
+            const uint32_t ndu = (O16 >> 4) * parts;
+            for (uint32_t it = (wave + 4 - (ndv & 3)) & 3; it < ndu; it += 4) {
+                const uint32_t ot = it / parts, part = it - ot * parts;
                 const uint32_t o = ot * 16 + l16;
                 const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
-                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                for (uint32_t dt = part; dt < dtiles; dt += parts) {
                     const uint32_t col0 = dt * 16 + g4;
-                    f4 acc = {0.f, 0.f, 0.f, 0.f};
+                    f4 acc = acc_old ? staged_bf16((const uint16_t *)Us + o * ldv, col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int kb = 0; kb < 2; ++kb) {
                         if ((uint32_t)kb >= nk32) break;
@@ -610,7 +625,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
                     } else {
-                        store_row<T, WB>(row, Us + o * ldv, col0, acc, rs);
+                        store_row<T, WB>(row, col0, acc, rs);
                     }
                 }
             }
@@ -621,9 +636,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 
         // ---- 5 on f32 MFMA (16x16x4), operands swapped as above -------------
         // dVin = G * Uout  (R16 x dp, K = O16: KQ blocks of 16 output rows)
+        const uint32_t ndv = (R16 >> 4) * parts;
         auto dv_rows = [&](auto kq) {
             constexpr int KQ = decltype(kq)::value;
-            for (uint32_t rt = 0; rt < (R16 >> 4); ++rt) {
+            for (uint32_t it = wave; it < ndv; it += 4) {
+                const uint32_t rt = it / parts, part = it - rt * parts;
                 const uint32_t r = rt * 16 + l16;
                 const LaneRow<T> row = lane_row(syn0, td.rin_tok, r, n_in, dp);
                 // G[r'][k], k = 16*q + 4*(lane/16) + i: one column of G^T per
@@ -634,7 +651,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 for (int q = 0; q < KQ; ++q)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) gx[4 * q + i] = ga[(16 * q + i) * GT];
-                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                for (uint32_t dt = part; dt < dtiles; dt += parts) {
                     const uint32_t col0 = dt * 16 + g4;
                     const T *ub = Us + g4 * ldv + dt * 16 + l16;
                     float ux[4 * KQ];
@@ -642,7 +659,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     for (int q = 0; q < KQ; ++q)
 #pragma unroll
                         for (int i = 0; i < 4; ++i) ux[4 * q + i] = load_elem(ub + (16 * q + i) * ldv);
-                    f4 acc = acc_old ? *(const f4 *)(Vs + r * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
+                    f4 acc = (acc_old && sizeof(T) == 4) ? *(const f4 *)(Vs + r * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int i = 0; i < 4 * KQ; ++i) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ux[i], gx[i], acc, 0, 0, 0);
                     if constexpr (DBG) {
@@ -650,7 +667,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
                     } else {
-                        store_row<T, WB>(row, Vs + r * ldv, col0, acc, rs);
+                        store_row<T, WB>(row, col0, acc, rs);
                     }
                 }
             }
@@ -660,7 +677,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         // dUout = G^T * Vin  (O16 x dp, K = R16: KQ blocks of 16 input rows)
         auto du_rows = [&](auto kq) {
             constexpr int KQ = decltype(kq)::value;
-            for (uint32_t ot = 0; ot < (O16 >> 4); ++ot) {
+            const uint32_t ndu = (O16 >> 4) * parts;
+            for (uint32_t it = (wave + 4 - (ndv & 3)) & 3; it < ndu; it += 4) {
+                const uint32_t ot = it / parts, part = it - ot * parts;
                 const uint32_t o = ot * 16 + l16;
                 const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
                 // G^T[o][k]: four consecutive words per 16-deep block, the
@@ -669,7 +688,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const float *ga = Gt + o * GT + g4;
 #pragma unroll
                 for (int q = 0; q < KQ; ++q) gx[q] = *(const f4 *)(ga + 16 * q);
-                for (uint32_t dt = wave; dt < dtiles; dt += 4) {
+                for (uint32_t dt = part; dt < dtiles; dt += parts) {
                     const uint32_t col0 = dt * 16 + g4;
                     const T *vb = Vs + g4 * ldv + dt * 16 + l16;
                     float vx[4 * KQ];
@@ -677,7 +696,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                     for (int q = 0; q < KQ; ++q)
 #pragma unroll
                         for (int i = 0; i < 4; ++i) vx[4 * q + i] = load_elem(vb + (16 * q + i) * ldv);
-                    f4 acc = acc_old ? *(const f4 *)(Us + o * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
+                    f4 acc = (acc_old && sizeof(T) == 4) ? *(const f4 *)(Us + o * ldv + col0) : f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int q = 0; q < KQ; ++q)
 #pragma unroll
@@ -688,7 +707,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
                     } else {
-                        store_row<T, WB>(row, Us + o * ldv, col0, acc, rs);
+                        store_row<T, WB>(row, col0, acc, rs);
                     }
                 }
             }

@@ -81,10 +81,14 @@ class BudgetModel(C.Structure):
 
 
 class TileShape(C.Structure):
-    """felog_tile_shape: SGNS tile rows, waves per SIMD, workgroups per CU
-    and (from the autotuner) the measured tile-kernel time."""
+    """felog_tile_shape: SGNS tile rows, waves per SIMD, workgroups per CU,
+    centre windows per wavefront, dim split (columns per wavefront item of
+    the update products) and (from the autotuner) the tile-kernel time."""
     _fields_ = [("rows", C.c_uint32), ("waves_per_simd", C.c_uint32), ("groups_per_cu", C.c_uint32),
-                ("tile_ms", C.c_float)]
+                ("windows_per_wave", C.c_uint32), ("dim_split", C.c_uint32), ("tile_ms", C.c_float)]
+
+
+SHAPE_FIELDS = ("rows", "waves_per_simd", "groups_per_cu", "windows_per_wave", "dim_split")
 
 
 _lib = None
@@ -355,26 +359,26 @@ class Felog:
     @staticmethod
     def _shape_dict(t):
         return {"rows": t.rows, "waves_per_simd": t.waves_per_simd, "groups_per_cu": t.groups_per_cu,
-                "tile_ms": t.tile_ms}
+                "windows_per_wave": t.windows_per_wave, "dim_split": t.dim_split, "tile_ms": t.tile_ms}
 
     def tile_shape(self):
         t = TileShape()
         self._check(self.lib.felog_tile_shape_get(self.ctx, C.byref(t)), "felog_tile_shape_get")
         return self._shape_dict(t)
 
-    def set_tile_shape(self, rows=0, waves_per_simd=0, groups_per_cu=0):
+    def set_tile_shape(self, rows=0, waves_per_simd=0, groups_per_cu=0, windows_per_wave=0, dim_split=0):
         """Zero fields take the defaults; set_tile_shape() restores all."""
-        t = TileShape(rows, waves_per_simd, groups_per_cu, 0.0)
+        t = TileShape(rows, waves_per_simd, groups_per_cu, windows_per_wave, dim_split, 0.0)
         self._check(self.lib.felog_tile_shape_set(self.ctx, C.byref(t)), "felog_tile_shape_set")
 
     def tune_sgns(self, sample=0, reps=2):
         """Time every tile shape on the current walk corpus (first `sample`
         sentences, 0: all) and install the fastest; returns every shape
         tried with its mean tile-kernel time."""
-        res = (TileShape * 64)()
+        res = (TileShape * 128)()
         n = C.c_uint32()
-        self._check(self.lib.felog_tune_sgns(self.ctx, sample, reps, res, 64, C.byref(n)), "felog_tune_sgns")
-        return [self._shape_dict(res[i]) for i in range(min(n.value, 64))]
+        self._check(self.lib.felog_tune_sgns(self.ctx, sample, reps, res, 128, C.byref(n)), "felog_tune_sgns")
+        return [self._shape_dict(res[i]) for i in range(min(n.value, 128))]
 
     def visits_reset(self):
         self._check(self.lib.felog_visits_reset(self.ctx), "felog_visits_reset")
