@@ -177,46 +177,56 @@ __device__ __forceinline__ LaneRow<T> lane_row(T *table, const uint32_t *ids, ui
     return R;
 }
 
-// Write the lane's four values (columns col0..col0+3 of its row; padding
-// columns past d hold zero deltas on zero rows, so whole vectors are
+// f32 table: the lane's four values (columns col0..col0+3 of its row;
+// padding columns past d hold zero deltas on zero rows, so whole vectors are
 // written).  Write-back: acc already holds the staged value plus the delta
-// (the products accumulate onto the staged row); f32 stores it, bf16 rounds
-// it stochastically (16 random bits per element) and stores 8 bytes.
-// Atomic: acc is the delta; f32 adds it, bf16 by two compare-and-swap pairs.
-// The rounding stream advances on every lane, live row or not.
+// (the products accumulate onto the staged row).  Atomic: acc is the delta.
 template <typename T, bool WB>
-__device__ __forceinline__ void store_row(const LaneRow<T> &R, uint32_t col0, const f4 &acc, uint32_t &rs) {
-    if constexpr (sizeof(T) == 4) {
-        if (!R.ok) return;
-        float *dst = (float *)R.p + col0;
-        if constexpr (WB) {
-            *(f4 *)dst = acc;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) unsafeAtomicAdd(dst + j, acc[j]);
-        }
+__device__ __forceinline__ void store_row(const LaneRow<T> &R, uint32_t col0, const f4 &acc) {
+    if (!R.ok) return;
+    float *dst = (float *)R.p + col0;
+    if constexpr (WB) {
+        *(f4 *)dst = acc;
     } else {
-        uint16_t *dst = (uint16_t *)R.p + col0;
-        const uint32_t b0 = round_bits(rs), b1 = round_bits(rs);
-        if constexpr (WB) {
-            uint2 nw;
-            nw.x = f32x2_to_bf16x2_sr(acc[0], acc[1], b0);
-            nw.y = f32x2_to_bf16x2_sr(acc[2], acc[3], b1);
-            if (R.ok) *(uint2 *)dst = nw;
-        } else {
-            if (!R.ok) return;
-            bf16_pair_add_sr(dst, acc[0], acc[1], b0);
-            bf16_pair_add_sr(dst + 2, acc[2], acc[3], b1);
-        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) unsafeAtomicAdd(dst + j, acc[j]);
     }
 }
 
-// The staged bf16 values of columns col0..col0+3 (the accumulator's start
-// for a write-back update)
-__device__ __forceinline__ f4 staged_bf16(const uint16_t *staged, uint32_t col0) {
-    const uint2 old = *(const uint2 *)(staged + col0);
-    return f4{__uint_as_float(old.x << 16), __uint_as_float(old.x & 0xFFFF0000u), __uint_as_float(old.y << 16),
-              __uint_as_float(old.y & 0xFFFF0000u)};
+// The staged bf16 values of columns c0..c0+7 (the accumulators' start for a
+// write-back update)
+__device__ __forceinline__ void staged_bf16x8(const uint16_t *staged, uint32_t c0, f4 &lo, f4 &hi) {
+    const uint4 v = *(const uint4 *)(staged + c0);
+    lo = f4{__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xFFFF0000u), __uint_as_float(v.y << 16),
+            __uint_as_float(v.y & 0xFFFF0000u)};
+    hi = f4{__uint_as_float(v.z << 16), __uint_as_float(v.z & 0xFFFF0000u), __uint_as_float(v.w << 16),
+            __uint_as_float(v.w & 0xFFFF0000u)};
+}
+
+// bf16 table: the lane's eight consecutive columns c0..c0+7 of its row.
+// Write-back: lo/hi hold staged value + delta, rounded stochastically (16
+// random bits per element) and stored as one 16-byte vector.  Atomic: the
+// deltas are added by compare-and-swap pairs.  The rounding stream advances
+// on every lane, live row or not.
+template <bool WB>
+__device__ __forceinline__ void store_row8(const LaneRow<uint16_t> &R, uint32_t c0, const f4 &lo, const f4 &hi,
+                                           uint32_t &rs) {
+    uint16_t *dst = R.p + c0;
+    const uint32_t b0 = round_bits(rs), b1 = round_bits(rs), b2 = round_bits(rs), b3 = round_bits(rs);
+    if constexpr (WB) {
+        uint4 nw;
+        nw.x = f32x2_to_bf16x2_sr(lo[0], lo[1], b0);
+        nw.y = f32x2_to_bf16x2_sr(lo[2], lo[3], b1);
+        nw.z = f32x2_to_bf16x2_sr(hi[0], hi[1], b2);
+        nw.w = f32x2_to_bf16x2_sr(hi[2], hi[3], b3);
+        if (R.ok) *(uint4 *)dst = nw;
+    } else {
+        if (!R.ok) return;
+        bf16_pair_add_sr(dst, lo[0], lo[1], b0);
+        bf16_pair_add_sr(dst + 2, lo[2], lo[3], b1);
+        bf16_pair_add_sr(dst + 4, hi[0], hi[1], b2);
+        bf16_pair_add_sr(dst + 6, hi[2], hi[3], b3);
+    }
 }
 
 // WPE: waves per SIMD the register budget is cut for (the LDS of one
@@ -563,10 +573,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         if constexpr (sizeof(T) == 2) {
             // ---- 5 on bf16 MFMA (16x16x32): G = hi + lo, two products each ----
             // Operands are swapped (first = table rows, second = G) so the
-            // products come out transposed: lane = one row, four columns.
+            // products come out transposed: lane = one row.  Columns go in
+            // pairs of 16-column blocks: the transposing loads point chunk q
+            // of block h of pair p at columns 32p + 8q + 4h, so after both
+            // blocks lane group g holds columns 32p + 8g .. 32p + 8g + 7 of
+            // its row, stored as one 16-byte vector (64 contiguous bytes per
+            // row and store instruction instead of 32).
             lds_u8 *l3 = (lds_u8 *)lds;
             const uint32_t grp = lane >> 4, jq = l16 >> 2, qq = l16 & 3;
             const uint32_t gb = (uint32_t)L.o_g, rb = (uint32_t)L.o_rows, rowb = ldv * 2;
+            const uint32_t npairs = dtiles >> 1, cq = 16 * qq, cg = 8 * grp;
             // dVin = G * Uout, K = all OMAX output rows (zero G past n_out)
             const uint32_t ndv = (R16 >> 4) * parts;
             for (uint32_t it = wave; it < ndv; it += 4) {
@@ -577,20 +593,27 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const uint32_t a0 = gb + (8 * grp + jq) * GT * 4 + (rt * 4 + qq) * 16;
                 const bf8 ghi = join8(tr_load(l3, a0), tr_load(l3, a0 + 16 * GT));
                 const bf8 glo = join8(tr_load(l3, a0 + 8), tr_load(l3, a0 + 16 * GT + 8));
-                for (uint32_t dt = part; dt < dtiles; dt += parts) {
-                    const uint32_t col0 = dt * 16 + g4;
-                    // Uout[o][16*dt + lane%16], transposed from the staged rows
-                    const uint32_t b0 = rb + (RM + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
-                    const bf8 uv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
-                    f4 acc = acc_old ? staged_bf16((const uint16_t *)Vs + r * ldv, col0) : f4{0.f, 0.f, 0.f, 0.f};
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, ghi, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(uv, glo, acc, 0, 0, 0);
+                for (uint32_t pp = part; pp < npairs; pp += parts) {
+                    const uint32_t c0 = 32 * pp + cg;
+                    // Uout[o][32p + 8q + 4h + i], transposed from the staged rows
+                    const uint32_t b0 = rb + (RM + 8 * grp + jq) * rowb + 64 * pp + cq;
+                    const bf8 u0 = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
+                    const bf8 u1 = join8(tr_load(l3, b0 + 8), tr_load(l3, b0 + 4 * rowb + 8));
+                    f4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = {0.f, 0.f, 0.f, 0.f};
+                    if constexpr (acc_old) staged_bf16x8((const uint16_t *)Vs + r * ldv, c0, x0, x1);
+                    x0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(u0, ghi, x0, 0, 0, 0);
+                    x1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(u1, ghi, x1, 0, 0, 0);
+                    x0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(u0, glo, x0, 0, 0, 0);
+                    x1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(u1, glo, x1, 0, 0, 0);
                     if constexpr (DBG) {
                         if (dbg_on)
-                            for (int j = 0; j < 4; ++j)
-                                if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
+                            for (int j = 0; j < 4; ++j) {
+                                float *dv = a.dbg_dv + (tile * RMAX + r) * d + c0;
+                                if (c0 + j < d) dv[j] = x0[j];
+                                if (c0 + 4 + j < d) dv[4 + j] = x1[j];
+                            }
                     } else {
-                        store_row<T, WB>(row, col0, acc, rs);
+                        store_row8<WB>(row, c0, x0, x1, rs);
                     }
                 }
             }
@@ -602,30 +625,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const uint32_t ot = it / parts, part = it - ot * parts;
                 const uint32_t o = ot * 16 + l16;
                 const LaneRow<T> row = lane_row(syn1, td.rout_tok, o, n_out, dp);
-                for (uint32_t dt = part; dt < dtiles; dt += parts) {
-                    const uint32_t col0 = dt * 16 + g4;
-                    f4 acc = acc_old ? staged_bf16((const uint16_t *)Us + o * ldv, col0) : f4{0.f, 0.f, 0.f, 0.f};
+                for (uint32_t pp = part; pp < npairs; pp += parts) {
+                    const uint32_t c0 = 32 * pp + cg;
+                    f4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = {0.f, 0.f, 0.f, 0.f};
+                    if constexpr (acc_old) staged_bf16x8((const uint16_t *)Us + o * ldv, c0, x0, x1);
 #pragma unroll
                     for (int kb = 0; kb < 2; ++kb) {
                         if ((uint32_t)kb >= nk32) break;
                         // G^T[o][r], r = 32*kb + 8*grp + s: two 4-row blocks
-                        // (read per column block: fewer live registers)
+                        // (read per column pair: fewer live registers)
                         const uint4 *gp = (const uint4 *)(lds + gb + (o * GT + 32 * kb + 8 * grp) * 4);
                         const uint4 x = gp[0], y = gp[1];
                         const bf8 ghi = __builtin_bit_cast(bf8, u4v{x.x, x.y, y.x, y.y});
                         const bf8 glo = __builtin_bit_cast(bf8, u4v{x.z, x.w, y.z, y.w});
-                        // Vin[r][16*dt + lane%16], transposed from the staged rows
-                        const uint32_t b0 = rb + (32 * kb + 8 * grp + jq) * rowb + (dt * 16 + 4 * qq) * 2;
-                        const bf8 vv = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, ghi, acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vv, glo, acc, 0, 0, 0);
+                        // Vin[r][32p + 8q + 4h + i], transposed from the staged rows
+                        const uint32_t b0 = rb + (32 * kb + 8 * grp + jq) * rowb + 64 * pp + cq;
+                        const bf8 v0 = join8(tr_load(l3, b0), tr_load(l3, b0 + 4 * rowb));
+                        const bf8 v1 = join8(tr_load(l3, b0 + 8), tr_load(l3, b0 + 4 * rowb + 8));
+                        x0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v0, ghi, x0, 0, 0, 0);
+                        x1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v1, ghi, x1, 0, 0, 0);
+                        x0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v0, glo, x0, 0, 0, 0);
+                        x1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(v1, glo, x1, 0, 0, 0);
                     }
                     if constexpr (DBG) {
                         if (dbg_on)
-                            for (int j = 0; j < 4; ++j)
-                                if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
+                            for (int j = 0; j < 4; ++j) {
+                                float *du = a.dbg_du + (tile * OMAX + o) * d + c0;
+                                if (c0 + j < d) du[j] = x0[j];
+                                if (c0 + 4 + j < d) du[4 + j] = x1[j];
+                            }
                     } else {
-                        store_row<T, WB>(row, col0, acc, rs);
+                        store_row8<WB>(row, c0, x0, x1, rs);
                     }
                 }
             }
@@ -667,7 +697,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_dv[(tile * RMAX + r) * d + col0 + j] = acc[j];
                     } else {
-                        store_row<T, WB>(row, col0, acc, rs);
+                        store_row<T, WB>(row, col0, acc);
                     }
                 }
             }
@@ -707,7 +737,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                             for (int j = 0; j < 4; ++j)
                                 if (col0 + j < d) a.dbg_du[(tile * OMAX + o) * d + col0 + j] = acc[j];
                     } else {
-                        store_row<T, WB>(row, col0, acc, rs);
+                        store_row<T, WB>(row, col0, acc);
                     }
                 }
             }
