@@ -44,7 +44,8 @@ import graphgen  # noqa: E402
 
 METRIC = "walk steps/s + SGNS words/s on 1 GPU; bf16+surrogate vs plain fp32 at equal AUC"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFS = 157.3
+# dense MFMA peaks: FP32 tables multiply on FP32 MFMA, BF16 tables on BF16 MFMA
+MFMA_PEAK_TFS = {"fp32": 157.3, "bf16": 2500.0}
 
 WORKLOADS = {
     # name: (generator, vertices, undirected edges, weight law)
@@ -195,6 +196,67 @@ def end_to_end(src, dst, w, n, args, model):
     return out
 
 
+def auc_study(src, dst, w, n, args):
+    """Held-out AUC of the trainers on the same corpora: two rounds of every
+    walker on the graph with 10% of its edges held out, then the cosine AUC
+    of syn0 (device scorer).  The walks depend only on the graph and the
+    seed, so every trainer sees the same two corpora:
+      tile trainer (HogBatch tiles) at its default base rate 0.0025 and at
+      the reference's 0.025 (compat_lr: ExponentialDecayLR(0.025, 0.1));
+      the reference's per-pair skip_gram_kernel rule on the GPU at 0.025;
+      the CPU restatement of that kernel (the oracle) at 0.025, with the
+      sigmoid table and the 1e8-entry unigram table."""
+    ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
+    pairs = np.concatenate([pos, neg]).astype(np.uint32)
+    labels = np.concatenate([np.ones(len(pos), np.uint8), np.zeros(len(neg), np.uint8)])
+    base = dict(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20, min_R=1 << 30,
+                seed=args.seed)
+    forms = (("tile_alpha_0.0025", dict()), ("tile_alpha_0.025", dict(compat_lr=1)),
+             ("skipgram_per_pair_gpu_alpha_0.025", dict(trainer=felog.TRAIN_SKIPGRAM, compat_lr=1)))
+    out, corpora, rp = {}, [], None
+    for name, kw in forms:
+        f = felog.Felog(**base, **kw)
+        f.set_edges(ts, td, tw, n, make_undirected=True)
+        f.build()
+        t0 = time.perf_counter()
+        for r in (1, 2):
+            f.walk_round(r)
+            if len(corpora) < r:
+                corpora.append(f.corpus()[:2])
+            f.train_round()
+        f.embeddings()  # waits for the device
+        out[name] = {"auc": f.link_auc(pairs, labels)[0], "seconds": time.perf_counter() - t0}
+        if rp is None:
+            rp, _, _ = f.csr(weights=False)
+        f.close()
+        print(f"[bench] auc study {name}: {out[name]['auc']:.4f}", file=sys.stderr, flush=True)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    deg = np.diff(rp.astype(np.int64)).astype(np.uint32)
+    order, rank, inv, vs, tw_ = orc.vocab(deg, 5)
+    c0 = orc.init_syn0(n, args.dim, rank)
+    c1 = np.zeros((n, args.dim), np.float32)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    for r, (off, tok) in enumerate(corpora, start=1):
+        alpha = np.float32(0.025 * np.exp(-0.1 * (r - 1)))
+        orc.w2v_train_mode(c0, c1, off, tok, deg, order, rank, inv, vs, tw_, mode=orc.W2V_SKIPGRAM, window=10,
+                           negative=5, sample=1e-3, alpha=alpha, table_size=100_000_000, threads=threads,
+                           seed=args.seed, rnd=r)
+    a, b = c0[pairs[:, 0]].astype(np.float64), c0[pairs[:, 1]].astype(np.float64)
+    cos = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-30)
+    out["skipgram_cpu_restatement_alpha_0.025"] = {
+        "auc": orc.auc(cos[labels == 1].astype(np.float32), cos[labels == 0].astype(np.float32)),
+        "seconds": time.perf_counter() - t0, "threads": threads}
+    print(f"[bench] auc study cpu: {out['skipgram_cpu_restatement_alpha_0.025']['auc']:.4f}", file=sys.stderr,
+          flush=True)
+    # what the held-out pairs look like: degrees of positive and negative endpoints
+    out["mean_endpoint_degree"] = {"held_out_edges": float(deg[pos].mean()), "non_edges": float(deg[neg].mean())}
+    out["definition"] = ("two rounds of every walker on the graph with 10% of edges held out (and as many "
+                         "non-edges), cosine of syn0, Mann-Whitney AUC; same corpora for every trainer")
+    return out
+
+
 def cpu_model():
     """CPU model, sockets and logical CPUs of this host (from lscpu)."""
     info = {}
@@ -264,6 +326,9 @@ def main():
     ap.add_argument("--e2e-no-auc", action="store_true", help="end-to-end runs on the whole graph, no AUC")
     ap.add_argument("--e2e-ablation", action="store_true",
                     help="also run the budgeted form with a constant budget instead of the network")
+    ap.add_argument("--auc-study", action="store_true",
+                    help="also compare the trainers' held-out AUC on the same corpora (tile at 0.0025 and 0.025, "
+                         "per-pair skip-gram on the GPU and the CPU restatement at 0.025)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default tile shape (profiling runs: one tile kernel variant)")
     args = ap.parse_args()
@@ -378,8 +443,9 @@ def main():
         "pairs_per_step": pairs / K,
         "walk_roofline": {"bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": walk_gbs / HBM_PEAK_GBS, "bytes_per_step": 24},
-        "sgns_roofline": {"hbm_GBs": sgns_gbs, "hbm_frac": sgns_gbs / HBM_PEAK_GBS, "fp32_TFs": sgns_tfs,
-                          "fp32_frac": sgns_tfs / FP32_PEAK_TFS,
+        "sgns_roofline": {"hbm_GBs": sgns_gbs, "hbm_frac": sgns_gbs / HBM_PEAK_GBS, "mfma_TFs": sgns_tfs,
+                          "mfma_peak_TFs": MFMA_PEAK_TFS[args.precision],
+                          "mfma_frac": sgns_tfs / MFMA_PEAK_TFS[args.precision],
                           "bytes_per_kept_word": sgns_bytes_per_word(args.dim, 5, 21, esize),
                           "flop_per_pair": 6 * 6 * args.dim},
         "roofline": roof,
@@ -399,6 +465,8 @@ def main():
         out["e2e"] = end_to_end(src, dst, w, n, args, model)
         out["e2e"]["budget_model"] = info
         print(f"[bench] end-to-end comparison ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    if args.auc_study:
+        out["auc_study"] = auc_study(src, dst, w, n, args)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(src, dst, w, n, args, args.seed, threads)

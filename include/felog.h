@@ -242,7 +242,7 @@ typedef struct felog_tile_shape {
     uint32_t dim_split;       /* columns of a 16-row block that one wavefront
                                  updates in the dVin / dUout products: dp,
                                  dp/2 or dp/4 (dp = d rounded up to 32; d =
-                                 128: 128, 64 or 32; default dp/4)          */
+                                 128: 128, 64 or 32; default dp)            */
     float tile_ms;            /* felog_tune_sgns: tile-kernel time          */
 } felog_tile_shape;
 felog_status felog_tile_shape_set(felog_ctx *ctx, const felog_tile_shape *shape);

@@ -896,7 +896,7 @@ felog_status felog_tile_shape_get(felog_ctx *c, felog_tile_shape *out) {
     TRY(c, tile_occupancy(c, out->rows, out->waves_per_simd, &per_cu));
     out->groups_per_cu = c->tile_gpc && (int)c->tile_gpc < per_cu ? c->tile_gpc : (uint32_t)per_cu;
     out->windows_per_wave = c->tile_wpw ? c->tile_wpw : 8u;
-    out->dim_split = c->dp / (c->tile_parts ? c->tile_parts : 4u);
+    out->dim_split = c->dp / (c->tile_parts ? c->tile_parts : 1u);
     out->tile_ms = 0.f;
     return FELOG_OK;
 }
@@ -962,7 +962,7 @@ felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t r
         const felog_tile_shape b = cand[fastest()];
         for (uint32_t wpw : {1u, 2u, 4u, 8u})
             for (uint32_t parts : {4u, 2u, 1u})
-                if (!(wpw == 8u && parts == 4u))  // the default, timed above
+                if (!(wpw == 8u && parts == 1u))  // the default, timed above
                     cand.push_back(felog_tile_shape{b.rows, b.waves_per_simd, b.groups_per_cu, wpw, c->dp / parts,
                                                     0.f});
         time_shapes(first);
@@ -979,7 +979,7 @@ felog_status felog_tune_sgns(felog_ctx *c, uint64_t sample_sentences, uint32_t r
     // shapes of the first sweep report the defaults they ran with
     for (size_t i = 0; i < first; ++i) {
         cand[i].windows_per_wave = 8u;
-        cand[i].dim_split = c->dp / 4u;
+        cand[i].dim_split = c->dp;
     }
     const size_t best = fastest();
     apply_shape(c, cand[best].rows, cand[best].waves_per_simd, cand[best].groups_per_cu,

@@ -151,7 +151,7 @@ struct felog_ctx {
     // SGNS tile shape (0: default): input rows 48/64, waves per SIMD the
     // kernel is compiled for, persistent workgroups per CU, centre windows
     // counted per wavefront (8), column parts of a row block in the update
-    // products (4)
+    // products (1)
     uint32_t tile_rows = 0, tile_wpe = 0, tile_gpc = 0, tile_wpw = 0, tile_parts = 0;
 
     // scratch for library primitives

@@ -313,7 +313,7 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
         // windows per wavefront 1, 2, 4 or 8: 64, 32, 16 or 8 lanes per window
         const uint32_t wpw = c->tile_wpw ? c->tile_wpw : 8u;
         a.win_shift = wpw >= 8 ? 3u : wpw >= 4 ? 4u : wpw >= 2 ? 5u : 6u;
-        a.col_parts = c->tile_parts ? c->tile_parts : 4u;
+        a.col_parts = c->tile_parts ? c->tile_parts : 1u;
     }
     a.compat_break = c->opt.compat_vocab_break;
     a.writeback = c->opt.update_mode != FELOG_UPDATE_ATOMIC;

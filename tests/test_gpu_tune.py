@@ -49,8 +49,8 @@ def shapes(f, prec):
 
 
 def window_split_shapes(dp):
-    """windows per wavefront x dim split, the default (8, dp/4) first"""
-    out = [(8, dp // 4)]
+    """windows per wavefront x dim split, the default (8, dp) first"""
+    out = [(8, dp)]
     for wpw in (1, 2, 4, 8):
         for ds in (dp // 4, dp // 2, dp):
             if (wpw, ds) != out[0]:
@@ -66,7 +66,7 @@ def test_every_tile_shape_trains_the_same_tables():
         cand = shapes(probe, felog.FP32)
     finally:
         probe.close()
-    cand = [(rows, wpe, gpc, 8, 16) for rows, wpe, gpc in cand]
+    cand = [(rows, wpe, gpc, 8, 64) for rows, wpe, gpc in cand]
     # windows per wavefront and dim split on the default rows / budget
     cand += [(32, 3, 0, wpw, ds) for wpw, ds in window_split_shapes(64)[1:]]
     print(f"{len(cand)} shapes: {cand}")
@@ -108,8 +108,8 @@ def test_tuner_restores_tables_and_installs_the_fastest(prec):
         key = lambda r: tuple(r[k] for k in felog.SHAPE_FIELDS)  # noqa: E731
         first = res[:len(expect)]
         # first sweep: every rows x budget x workgroups at the default
-        # windows per wavefront (8) and dim split (dp/4 = 16)
-        assert [key(r) for r in first] == [e + (8, 16) for e in expect]
+        # windows per wavefront (8) and dim split (dp = 64)
+        assert [key(r) for r in first] == [e + (8, 64) for e in expect]
         # second sweep: windows x split at the first sweep's fastest
         b = min(first, key=lambda r: r["tile_ms"])
         assert [key(r) for r in res[len(expect):]] == \
