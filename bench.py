@@ -201,17 +201,18 @@ def auc_study(src, dst, w, n, args):
     walker on the graph with 10% of its edges held out, then the cosine AUC
     of syn0 (device scorer).  The walks depend only on the graph and the
     seed, so every trainer sees the same two corpora:
-      tile trainer (HogBatch tiles) at its default base rate 0.0025 and at
-      the reference's 0.025 (compat_lr: ExponentialDecayLR(0.025, 0.1));
-      the reference's per-pair skip_gram_kernel rule on the GPU at 0.025;
-      the CPU restatement of that kernel (the oracle) at 0.025, with the
-      sigmoid table and the 1e8-entry unigram table."""
+      tile trainer (HogBatch tiles), the reference's per-pair
+      skip_gram_kernel rule on the GPU, and the CPU restatement of that
+      kernel (the oracle, with the sigmoid table and the 1e8-entry unigram
+      table), each at the tile trainer's base rate 0.0025 and at the
+      reference's 0.025 (ExponentialDecayLR(0.025, 0.1))."""
     ts, td, tw, pos, neg = graphgen.lp_split(src, dst, w, n, frac=0.1, seed=3)
     pairs = np.concatenate([pos, neg]).astype(np.uint32)
     labels = np.concatenate([np.ones(len(pos), np.uint8), np.zeros(len(neg), np.uint8)])
     base = dict(size=args.dim, window=10, negative=5, batch_size=21, sample=1e-3, min_L=20, min_R=1 << 30,
                 seed=args.seed)
-    forms = (("tile_alpha_0.0025", dict()), ("tile_alpha_0.025", dict(compat_lr=1)),
+    forms = (("tile_alpha_0.0025", dict(alpha=0.0025)), ("tile_alpha_0.025", dict(compat_lr=1)),
+             ("skipgram_per_pair_gpu_alpha_0.0025", dict(trainer=felog.TRAIN_SKIPGRAM, alpha=0.0025)),
              ("skipgram_per_pair_gpu_alpha_0.025", dict(trainer=felog.TRAIN_SKIPGRAM, compat_lr=1)))
     out, corpora, rp = {}, [], None
     for name, kw in forms:
@@ -234,22 +235,22 @@ def auc_study(src, dst, w, n, args):
     import orc
     deg = np.diff(rp.astype(np.int64)).astype(np.uint32)
     order, rank, inv, vs, tw_ = orc.vocab(deg, 5)
-    c0 = orc.init_syn0(n, args.dim, rank)
-    c1 = np.zeros((n, args.dim), np.float32)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    for r, (off, tok) in enumerate(corpora, start=1):
-        alpha = np.float32(0.025 * np.exp(-0.1 * (r - 1)))
-        orc.w2v_train_mode(c0, c1, off, tok, deg, order, rank, inv, vs, tw_, mode=orc.W2V_SKIPGRAM, window=10,
-                           negative=5, sample=1e-3, alpha=alpha, table_size=100_000_000, threads=threads,
-                           seed=args.seed, rnd=r)
-    a, b = c0[pairs[:, 0]].astype(np.float64), c0[pairs[:, 1]].astype(np.float64)
-    cos = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-30)
-    out["skipgram_cpu_restatement_alpha_0.025"] = {
-        "auc": orc.auc(cos[labels == 1].astype(np.float32), cos[labels == 0].astype(np.float32)),
-        "seconds": time.perf_counter() - t0, "threads": threads}
-    print(f"[bench] auc study cpu: {out['skipgram_cpu_restatement_alpha_0.025']['auc']:.4f}", file=sys.stderr,
-          flush=True)
+    for base_rate in (0.0025, 0.025):
+        c0 = orc.init_syn0(n, args.dim, rank)
+        c1 = np.zeros((n, args.dim), np.float32)
+        t0 = time.perf_counter()
+        for r, (off, tok) in enumerate(corpora, start=1):
+            alpha = np.float32(base_rate * np.exp(-0.1 * (r - 1)))
+            orc.w2v_train_mode(c0, c1, off, tok, deg, order, rank, inv, vs, tw_, mode=orc.W2V_SKIPGRAM, window=10,
+                               negative=5, sample=1e-3, alpha=alpha, table_size=100_000_000, threads=threads,
+                               seed=args.seed, rnd=r)
+        a, b = c0[pairs[:, 0]].astype(np.float64), c0[pairs[:, 1]].astype(np.float64)
+        cos = (a * b).sum(1) / (np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1) + 1e-30)
+        name = f"skipgram_cpu_restatement_alpha_{base_rate}"
+        out[name] = {"auc": orc.auc(cos[labels == 1].astype(np.float32), cos[labels == 0].astype(np.float32)),
+                     "seconds": time.perf_counter() - t0, "threads": threads}
+        print(f"[bench] auc study {name}: {out[name]['auc']:.4f}", file=sys.stderr, flush=True)
     # what the held-out pairs look like: degrees of positive and negative endpoints
     out["mean_endpoint_degree"] = {"held_out_edges": float(deg[pos].mean()), "non_edges": float(deg[neg].mean())}
     out["definition"] = ("two rounds of every walker on the graph with 10% of edges held out (and as many "

@@ -195,13 +195,16 @@ def _auc(emb, pos, neg):
 
 def test_learned_budget_against_the_plain_rule():
     """Train the network on the plain rule's budgets of one R-MAT graph and
-    run a second R-MAT graph with it: held-out AUC within 0.01 of the plain
-    rule's, visit entropy within 2%, evaluation only for vertices outside
-    the network's feature box, and a corpus within 10% of the plain rule's
-    on the training graph.  On the second graph the plain rule's stop ratio
-    (unconverged count shrinking by less than 30%) can fire one round
-    earlier or later, which moves its corpus by a quarter: there the bound
-    is 30%."""
+    run a second R-MAT graph with it (felog_run against felog_run_budget):
+    held-out AUC within 0.005 of the plain rule's either side, visit entropy
+    within 2%, no evaluation (every vertex is inside the network's feature
+    box), less device time than the plain rule's (at 20k vertices both run
+    three launch-bound rounds of ~1 ms, so the wall time is not compared
+    here; tests/test_gpu_zy_end_to_end.py compares it at 2.24M vertices),
+    and a corpus within 10% of the plain rule's on the training graph.  On the second graph the
+    plain rule's stop ratio (unconverged count shrinking by less than 30%)
+    can fire one round earlier or later, which moves its corpus by a
+    quarter: there the bound is 30%."""
     n = 20_000
     a = graphgen.rmat(n, 100_000, seed=21)
     fa = ctx(*a, n, alpha=0.025)
@@ -224,28 +227,31 @@ def test_learned_budget_against_the_plain_rule():
             f.visits_reset()
             t0 = time.perf_counter()
             if mode == "plain":
-                cnt, length, corpora, _ = walk_budget.plain_budgets(f, max_rounds=12)
-                tokens = sum(int(o[-1]) for o, _ in corpora)
+                f.run()
             else:
                 f.budget_set(model)
                 bc, bl, rounds = f.budget()
                 print(f"learned budget: rounds {rounds}, counts {np.bincount(bc)[:8]}, "
                       f"lengths {np.unique(bl)[:8]}, plain-rule fallback {(bc == 0).sum()}")
                 f.run_budget()
-                tokens = f.visit_stats()["visits"]
             wall = time.perf_counter() - t0
+            tokens = f.visit_stats()["visits"]
             st = f.stats()
             emb, _ = f.embeddings()
             vs = f.visit_stats()
             res[mode] = dict(auc=_auc(emb, pos, neg), tokens=tokens, entropy=vs["entropy"], kl=vs["kl"],
-                             wall=wall, eval_ms=st["eval_ms_total"], walk_ms=st["walk_ms_total"])
+                             wall=wall, eval_ms=st["eval_ms_total"], walk_ms=st["walk_ms_total"],
+                             device_ms=st["walk_ms_total"] + st["train_ms_total"] + st["eval_ms_total"])
         finally:
             f.close()
     print({k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in res.items()})
     p, l_ = res["plain"], res["learned"]
-    assert l_["auc"] >= p["auc"] - 0.01
+    assert abs(l_["auc"] - p["auc"]) <= 0.005
+    print(f"wall time: plain {p['wall'] * 1e3:.2f} ms, learned budget {l_['wall'] * 1e3:.2f} ms; device time "
+          f"{p['device_ms']:.3f} vs {l_['device_ms']:.3f} ms")
+    assert l_["device_ms"] < p["device_ms"]
     print(f"plain-rule corpus on the training graph: {tokens_a} tokens in {len(corpora_a)} rounds")
     assert abs(l_["tokens"] - tokens_a) <= 0.1 * tokens_a
     assert abs(l_["tokens"] - p["tokens"]) <= 0.3 * p["tokens"]
     assert abs(l_["entropy"] - p["entropy"]) <= 0.02 * p["entropy"]
-    assert l_["eval_ms"] <= p["eval_ms"]
+    assert l_["eval_ms"] == 0.0 < p["eval_ms"]
