@@ -182,10 +182,12 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             }
             TileDesc *td = (TileDesc *)(a.tiles + slot * kDescBytes);
             // distinct input rows over positions [lo, hi)
+            uint32_t first_in;
             {
                 const bool valid = lane < npos;
                 const uint32_t t = valid ? stok[lo + lane] : 0xFFFFFFFFu;
                 const uint32_t first = first_lane_of(t, valid);
+                first_in = first;
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
@@ -206,7 +208,21 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 const bool valid = lane < nl;
                 uint32_t t = 0xFFFFFFFFu;
                 if (valid) t = lane < nc ? stok[c0 + lane] : sneg[lane - nc];
-                const uint32_t first = first_lane_of(t, valid);
+                uint32_t first;
+                if (c0 == lo && nc == npos) {
+                    // the centres are the input positions (a tile over the
+                    // whole window span): their first lanes are the input
+                    // pass's; each negative takes the lowest lane holding
+                    // its row
+                    first = lane < nc ? first_in : lane;
+                    for (uint32_t q = nc; q < nl; ++q) {
+                        const uint32_t tq = __builtin_amdgcn_readlane(t, q);
+                        const uint64_t eq = __builtin_amdgcn_ballot_w64(valid && t == tq);
+                        if (lane == q) first = (uint32_t)__builtin_ctzll(eq);
+                    }
+                } else {
+                    first = first_lane_of(t, valid);
+                }
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
                 const uint32_t idx = (uint32_t)__popcll(bal & lt_mask);
