@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--e2e-no-auc", action="store_true", help="end-to-end runs on the whole graph, no AUC")
     ap.add_argument("--e2e-ablation", action="store_true",
                     help="also run the budgeted form with a constant budget instead of the network")
+    ap.add_argument("--lost-updates", action="store_true",
+                    help="after the timed rounds, one more round with the tile trainer's lost-update tracking on")
     ap.add_argument("--auc-study", action="store_true",
                     help="also compare the trainers' held-out AUC on the same corpora (tile at 0.0025 and 0.025, "
                          "per-pair skip-gram on the GPU and the CPU restatement at 0.025)")
@@ -458,6 +460,18 @@ def main():
             "fields": ", ".join(felog.SHAPE_FIELDS) + ", tile kernel ms on the warm-up corpus"},
         "cpu_baseline": None,
     }
+    if args.lost_updates:
+        # one more round with per-row versions: write-backs that replaced
+        # another tile's update (not part of the timed region)
+        f.track_lost_updates(True)
+        r = args.warmup + args.steps + 2
+        f.walk_round(r)
+        f.train_round()
+        s = f.stats()
+        out["lost_updates"] = {"round": r, "rows_written": s["train_rows_written"], "rows_lost": s["train_rows_lost"],
+                               "fraction": s["train_rows_lost"] / max(s["train_rows_written"], 1),
+                               "tile_ms_with_tracking": s["sgns_tile_ms"]}
+        f.track_lost_updates(False)
     f.close()
     if not args.no_e2e:
         t0 = time.perf_counter()

@@ -157,6 +157,11 @@ typedef struct felog_stats {
     double graph_ms_total;           /* captured rounds (felog_run_graph),
                                         HIP events around each replay       */
     uint64_t walk_tokens_total;      /* corpus tokens of all walk rounds     */
+    uint64_t train_rows_written;     /* last round, lost-update tracking on:
+                                        table rows the tiles wrote back     */
+    uint64_t train_rows_lost;        /* of those, writes that replaced another
+                                        tile's update to the same row (the
+                                        row changed after the tile read it) */
 } felog_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -400,6 +405,15 @@ felog_status felog_visits_reset(felog_ctx *ctx);
  * out[1] = Shannon entropy -sum q ln q (nats), out[2] = vertices with p > 0
  * and no visit, out[3] = visits counted. */
 felog_status felog_visit_stats(felog_ctx *ctx, double out[4]);
+/* Lost-update tracking of the HogBatch tile trainer's write-back (no
+ * reference counterpart: word2vec's unsynchronised CPU threads, the same
+ * Hogwild scheme, lose such updates uncounted).  With on = 1 every table
+ * row gets a version counter (8 bytes per vertex); a tile reads a row's
+ * version when it gathers the row and increments it when it writes the row
+ * back, and a write-back that finds the version moved replaces another
+ * tile's update.  felog_stats' train_rows_written / train_rows_lost report
+ * the last round's counts.  on = 0 frees the counters. */
+felog_status felog_track_lost_updates(felog_ctx *ctx, int on);
 
 /* ---- test hook --------------------------------------------------------- */
 /* Runs the SGNS kernels on a caller-supplied corpus with updates diverted:

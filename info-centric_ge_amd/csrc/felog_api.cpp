@@ -75,7 +75,7 @@ void free_all(felog_ctx *c) {
                     c->d_counters, c->d_off, c->d_tok, c->d_sent_id, c->d_tiles, c->d_tmp,
                     c->d_budget_cnt, c->d_budget_len, c->d_model, c->d_feat, c->d_visits, c->d_vstats,
                     c->d_w2v_ftok, c->d_w2v_kcnt, c->d_w2v_blk, c->d_w2v_table, c->d_w2v_exp, c->d_scol,
-                    c->d_cn, c->hubs.hub, c->hubs.cstart, c->d_feat_part, c->d_budget_sum};
+                    c->d_cn, c->hubs.hub, c->hubs.cstart, c->d_feat_part, c->d_budget_sum, c->d_row_ver};
     for (void *p : ptrs) dfree(p);
     drop_round_graph(c);
     dfree(c->d_rs);
@@ -356,6 +356,8 @@ static felog_status train_device_corpus(felog_ctx *c, const uint64_t *d_off, con
     c->st.train_kept = c->h_counters[C_KEPT];
     c->st.train_pairs = c->h_counters[C_PAIRS];
     c->st.train_groups = c->h_counters[C_GROUPS];
+    c->st.train_rows_written = c->h_counters[C_ROWS_WB];
+    c->st.train_rows_lost = c->h_counters[C_ROWS_LOST];
     c->st.train_kernel_ms = ms[0];
     c->st.sgns_prep_ms = ms[1];
     c->st.sgns_tile_ms = ms[2];
@@ -775,6 +777,8 @@ felog_status felog_run_graph(felog_ctx *c, uint32_t max_rounds) {
         c->st.train_kept = k[C_KEPT];
         c->st.train_pairs = k[C_PAIRS];
         c->st.train_groups = k[C_GROUPS];
+        c->st.train_rows_written = k[C_ROWS_WB];
+        c->st.train_rows_lost = k[C_ROWS_LOST];
         if (visits) c->visits_total += h.n_tokens;
         if (k[C_OVF_FAIL])
             return fail(c, FELOG_ERR_CAPACITY,
@@ -808,6 +812,21 @@ felog_status felog_visits_reset(felog_ctx *c) {
     TRY(c, hipMemsetAsync(c->d_visits, 0, (uint64_t)c->V * 4, c->stream));
     TRY(c, hipStreamSynchronize(c->stream));
     c->visits_total = 0;
+    return FELOG_OK;
+}
+
+felog_status felog_track_lost_updates(felog_ctx *c, int on) {
+    felog_status s = need_built(c);
+    if (s) return s;
+    if (!on) {
+        dfree(c->d_row_ver);
+        c->d_row_ver = nullptr;
+    } else if (!c->d_row_ver) {
+        TRY(c, hipMalloc(&c->d_row_ver, 2ull * c->V * 4));
+        TRY(c, hipMemsetAsync(c->d_row_ver, 0, 2ull * c->V * 4, c->stream));
+        TRY(c, hipStreamSynchronize(c->stream));
+    }
+    drop_round_graph(c);  // captured rounds hold the tracking pointer
     return FELOG_OK;
 }
 

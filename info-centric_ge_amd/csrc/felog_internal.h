@@ -177,6 +177,7 @@ struct felog_ctx {
     bool budget_recheck = false;         // score budgeted vertices once at min_R
     // corpus visit histogram (cumulative over walk rounds while tracked)
     uint32_t *d_visits = nullptr;        // V, indexed by vocabulary rank
+    uint32_t *d_row_ver = nullptr;       // 2V row versions (lost-update tracking; null: off)
     uint64_t visits_total = 0;
     double *d_vstats = nullptr;          // 3 doubles
 
@@ -215,7 +216,9 @@ enum : int {
     C_BIGTILES = 14,   // SGNS tiles with more than 32 input rows
     C_CURSOR = 15,     // SGNS tile cursor (zero between launches)
     C_DONE = 16,       // SGNS tile workgroups finished (zero between launches)
-    kNumCounters = 18
+    C_ROWS_WB = 17,    // SGNS rows written back (lost-update tracking on)
+    C_ROWS_LOST = 18,  // of those, writes over another tile's update to the row
+    kNumCounters = 19
 };
 
 // Device memory for the build's temporaries: one block taken at the start of

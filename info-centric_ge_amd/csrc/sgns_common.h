@@ -63,6 +63,10 @@ struct Params {
     // tiles of this launch: n_in in [nin_lo, nin_hi] (a 32-row kernel takes
     // the small tiles, a 48/64-row one the rest)
     uint32_t nin_lo, nin_hi;
+    // lost-update tracking (null: off): one version per row of the joint
+    // table allocation (syn0 rows, then syn1neg rows), read when a tile
+    // gathers the row and incremented when it writes the row back
+    uint32_t *row_ver;
     // tile shape: log2 of the lanes that count one centre's window (3: eight
     // windows per wavefront), and the column parts of a row block in the
     // update products (a wavefront's item is one row block's columns

@@ -343,6 +343,8 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
     a.dbg_du = dbg ? dbg->du : nullptr;
     a.tiles = (unsigned char *)tiles;
     a.tiles_cap = tiles_cap;
+    // lost updates are a property of write-back tiles that train the tables
+    a.row_ver = (dbg || !a.writeback) ? nullptr : c->d_row_ver;
     // the sentence count is read on the device: a fixed grid strides over it
     k_sgns_prep<<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
     HIP_TRY(hipGetLastError());
@@ -367,6 +369,7 @@ hipError_t run_sgns(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, 
 hipError_t run_sgns_into(felog_ctx *c, const uint64_t *d_off, const uint32_t *d_tok, const SgnsDebug *dbg,
                          float *ms, void *tiles, uint64_t tiles_cap) {
     HIP_TRY(hipMemsetAsync(c->d_counters + C_WORDS, 0, 8 * (C_BIGTILES - C_WORDS + 1), c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_counters + C_ROWS_WB, 0, 8 * 2, c->stream));
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     HIP_TRY(sgns_enqueue_into(c, d_off, d_tok, dbg, c->ev1, tiles, tiles_cap));
     HIP_TRY(hipEventRecord(c->ev2, c->stream));

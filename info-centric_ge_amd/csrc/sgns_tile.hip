@@ -70,16 +70,17 @@ __host__ __device__ inline uint32_t tile_wrows(uint32_t k) {
 // LDS: two descriptor slots (this tile's, the next one's) | RM input rows
 // then OMAX output rows (pitch ldv) | the OMAX-row count / G^T tile, then
 // one row of context counts per group (same pitch) | per-output-row masks
-// of negative slots | the tile cursor's broadcast word.  At k = 5 and
+// of negative slots | the staged rows' versions (lost-update tracking) |
+// the tile cursor's broadcast word.  At k = 5 and
 // d = 128 the f32 32-row layout (40.5 KB)
 // fits four workgroups in a CU's 160 KB, the bf16 one six.
 struct Lds {
     uint32_t ldv;    // row pitch of staged rows, in elements
     uint32_t wrows;  // rows of per-group context counts
-    size_t o_desc, o_rows, o_g, o_mask, o_grab, total;
+    size_t o_desc, o_rows, o_g, o_mask, o_ver, o_grab, total;
 };
 
-__host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax, uint32_t k) {
+__host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t rmax, uint32_t k, bool trk = false) {
     Lds L;
     // skew between rows: 16 bytes for f32; 32 bytes for bf16, whose
     // transposing loads conflict at 16 (the bf16 tile runs 21% faster)
@@ -90,6 +91,7 @@ __host__ __device__ inline Lds lds_layout(uint32_t dp, uint32_t esize, uint32_t 
     L.o_rows = o; o += (size_t)(rmax + OMAX) * L.ldv * esize;
     L.o_g = o;    o += (size_t)(OMAX + L.wrows) * gt_pitch(esize, rmax) * 4;
     L.o_mask = o; o += (size_t)OMAX * 4;
+    L.o_ver = o;  o += trk ? (size_t)(rmax + OMAX) * 4 : 0;
     L.o_grab = o; o += 8;
     L.total = o;
     return L;
@@ -235,7 +237,9 @@ __device__ __forceinline__ void store_row8(const LaneRow<uint16_t> &R, uint32_t 
 // LDS offset is an immediate and the column loops unroll), 0: from a.dp.
 // DBG: the test hook (products diverted to the debug buffers, tables left
 // alone); the production kernels carry none of its code.
-template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
+// TRK: lost-update tracking (a.row_ver), likewise compiled only into the
+// kernels that run with it.
+template <typename T, int RM, bool WB, int WPE, int DP, bool DBG, bool TRK>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_sgns_tile(Params a) {
     constexpr uint32_t GT = gt_pitch(sizeof(T), RM);  // pitch of the count / G^T tile (words)
     constexpr uint32_t EPV = 16 / sizeof(T);          // elements per 16-byte vector
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     // d = 128 (4), else the first NB of the tile's vectors
     constexpr int NB = sizeof(T) == 4 ? 8 : ((DP == 128 && RM == 32) ? 4 : 5);
     extern __shared__ __align__(16) unsigned char lds[];
-    const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM, a.negative);
+    const Lds L = lds_layout(DP ? (uint32_t)DP : a.dp, sizeof(T), RM, a.negative, TRK);
     const uint32_t ldv = L.ldv;
     T *Vs = (T *)(lds + L.o_rows);
     T *Us = Vs + RM * ldv;
@@ -251,6 +255,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     uint32_t *cnt = (uint32_t *)Gt;  // counts, overwritten in place by G^T
     uint32_t *wcnt = cnt + OMAX * GT;  // per-group context counts W[g][r]
     uint32_t *negmask = (uint32_t *)(lds + L.o_mask);
+    uint32_t *verl = (uint32_t *)(lds + L.o_ver);  // staged row -> version at its gather
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t l16 = lane & 15, g4 = 4 * (lane >> 4);
@@ -280,6 +285,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     constexpr bool acc_old = WB && !DBG;
     // stochastic-rounding stream of this thread (bf16 tables), seeded per tile
     uint32_t rs = 0;
+    // lost-update tracking: rows written back, and of those the ones whose
+    // version moved since this tile gathered them (another tile wrote the
+    // row in between, and this write-back replaces that update).  Lanes
+    // 0-15 take the 16 rows of a row block, once per row (column part 0).
+    uint64_t rows_wb = 0, rows_lost = 0;
+    auto track_rows = [&](bool live, uint32_t vidx, uint32_t lrow) {
+        const bool mine = live && lane < 16;
+        bool lost = false;
+        if (mine) lost = atomicAdd(&a.row_ver[vidx], 1u) != verl[lrow];
+        rows_wb += (uint64_t)__popcll(__ballot(mine));
+        rows_lost += (uint64_t)__popcll(__ballot(lost));
+    };
 
     // bf16 products contract over whole 32-row blocks, reading staged rows
     // past a tile's own (rows of earlier tiles, finite); clear them once
@@ -365,6 +382,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 #pragma unroll
     for (int j = 0; j < NB; ++j) advance(row_x, c_x);
     uint4 buf[NB];
+    uint32_t my_ver = 0;  // version of the staged row this thread tracks (row_ver)
     // this thread's first NB vectors of the tile in slot s (padding rows 0):
     // slots and LDS reads of the row ids first, then the loads
     auto gather = [&](uint32_t s) {
@@ -388,6 +406,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             const bool live = tix[j] != 0xFFFFFFFFu;
             const uint32_t id = tk[live ? tix[j] : 0u];
             grow[j] = live ? id + (tix[j] >= (uint32_t)RMAX ? a.V : 0u) : 0u;
+        }
+        // lost-update tracking: thread s reads the version of staged row s
+        if constexpr (TRK) {
+            const uint32_t rr = tid - R16;
+            my_ver = 0xFFFFFFFFu;
+            if (tid < n_in) my_ver = a.row_ver[tk[tid]];
+            else if (tid >= R16 && rr < n_out) my_ver = a.row_ver[a.V + tk[RMAX + rr]];
         }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -477,6 +502,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 val = *(const uint4 *)((in ? syn0 : syn1) + ((uint64_t)toks[in ? r : RMAX + r] * dp + c * EPV));
             *(uint4 *)(Vs + (in ? row : RM + r) * ldv + c * EPV) = val;
         }
+        if (TRK && tid < nrows) verl[tid < R16 ? tid : RM + tid - R16] = my_ver;
         if (tid < (uint32_t)kDescVec) ((uint4 *)&slot(s ^ 1))[tid] = pre;
         if (pending && tid == 0) *grab_lds = grab;
         __syncthreads();
@@ -616,6 +642,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                         store_row8<WB>(row, c0, x0, x1, rs);
                     }
                 }
+                if (TRK && part == 0) track_rows(r < n_in, td.rin_tok[r < n_in ? r : 0], r);
             }
             // dUout = G^T * Vin, K = R16 rounded up to 32 (zero G past n_in;
             // input rows past RM are output rows, finite)
@@ -658,6 +685,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                         store_row8<WB>(row, c0, x0, x1, rs);
                     }
                 }
+                if (TRK && part == 0)
+                    track_rows(o < n_out, a.V + td.rout_tok[o < n_out ? o : 0], RM + o);
             }
             ++my_tiles;
             __syncthreads();  // the next tile reuses the LDS
@@ -700,6 +729,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                         store_row<T, WB>(row, col0, acc);
                     }
                 }
+                if (TRK && part == 0) track_rows(r < n_in, td.rin_tok[r < n_in ? r : 0], r);
             }
         };
         if (O16 > 16) dv_rows(std::integral_constant<int, 2>{});
@@ -740,6 +770,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                         store_row<T, WB>(row, col0, acc);
                     }
                 }
+                if (TRK && part == 0)
+                    track_rows(o < n_out, a.V + td.rout_tok[o < n_out ? o : 0], RM + o);
             }
         };
         switch (R16 >> 4) {
@@ -764,17 +796,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     }
     for (int o = 32; o > 0; o >>= 1) my_pairs += __shfl_xor(my_pairs, o);
     if (lane == 0) atomicAdd(&a.counters[C_PAIRS], my_pairs);
+    if (TRK && lane == 0) {
+        atomicAdd(&a.counters[C_ROWS_WB], (unsigned long long)rows_wb);
+        atomicAdd(&a.counters[C_ROWS_LOST], (unsigned long long)rows_lost);
+    }
 }
 
 template <typename T> constexpr int kWpeLow = sizeof(T) == 4 ? 2 : 3;
 
-template <typename T, int RM, bool WB, int WPE, int DP, bool DBG = false>
+template <typename T, int RM, bool WB, int WPE, int DP, bool DBG = false, bool TRK = false>
 hipError_t occupancy_dp(felog_ctx *c, int *per_cu) {
-    const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative);
-    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE, DP, DBG>,
+    const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative, TRK);
+    hipError_t e = hipFuncSetAttribute((const void *)k_sgns_tile<T, RM, WB, WPE, DP, DBG, TRK>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.total);
     if (e != hipSuccess) return e;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE, DP, DBG>, kBlock,
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_sgns_tile<T, RM, WB, WPE, DP, DBG, TRK>, kBlock,
                                                      L.total);
     if (*per_cu < 1) *per_cu = 1;
     return e;
@@ -787,15 +823,15 @@ hipError_t occupancy(felog_ctx *c, int *per_cu) {
 
 // Persistent grid of gpc workgroups per CU (0 or more than fit: as many as
 // fit); *blocks receives the grid.
-template <typename T, int RM, bool WB, int WPE, int DP, bool DBG>
+template <typename T, int RM, bool WB, int WPE, int DP, bool DBG, bool TRK = false>
 hipError_t launch_dp(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks) {
     int per_cu = 0;
-    hipError_t e = occupancy_dp<T, RM, WB, WPE, DP, DBG>(c, &per_cu);
+    hipError_t e = occupancy_dp<T, RM, WB, WPE, DP, DBG, TRK>(c, &per_cu);
     if (e != hipSuccess) return e;
     if (gpc && (int)gpc < per_cu) per_cu = (int)gpc;
     *blocks = (unsigned)per_cu * c->cu_count;
-    const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative);
-    k_sgns_tile<T, RM, WB, WPE, DP, DBG><<<*blocks, kBlock, L.total, c->stream>>>(a);
+    const Lds L = lds_layout(c->dp, sizeof(T), RM, c->opt.negative, TRK);
+    k_sgns_tile<T, RM, WB, WPE, DP, DBG, TRK><<<*blocks, kBlock, L.total, c->stream>>>(a);
     return hipGetLastError();
 }
 
@@ -804,6 +840,11 @@ hipError_t launch(felog_ctx *c, const Params &a, uint32_t gpc, unsigned *blocks)
     if (a.dbg_max)  // the test hook: one register budget per table type
         return c->dp == 128 ? launch_dp<T, RM, WB, kWpeLow<T> + 1, 128, true>(c, a, gpc, blocks)
                             : launch_dp<T, RM, WB, kWpeLow<T> + 1, 0, true>(c, a, gpc, blocks);
+    if constexpr (WB) {
+        if (a.row_ver)  // lost-update tracking (write-back only)
+            return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128, false, true>(c, a, gpc, blocks)
+                                : launch_dp<T, RM, WB, WPE, 0, false, true>(c, a, gpc, blocks);
+    }
     return c->dp == 128 ? launch_dp<T, RM, WB, WPE, 128, false>(c, a, gpc, blocks)
                         : launch_dp<T, RM, WB, WPE, 0, false>(c, a, gpc, blocks);
 }

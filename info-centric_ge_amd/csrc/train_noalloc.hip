@@ -107,6 +107,8 @@ extern "C" felog_status felog_train_corpus_noalloc(felog_ctx *c, const uint64_t 
     c->st.train_kept = c->h_counters[C_KEPT];
     c->st.train_pairs = c->h_counters[C_PAIRS];
     c->st.train_groups = c->h_counters[C_GROUPS];
+    c->st.train_rows_written = c->h_counters[C_ROWS_WB];
+    c->st.train_rows_lost = c->h_counters[C_ROWS_LOST];
     c->st.train_kernel_ms = ms[0];
     c->st.sgns_prep_ms = ms[1];
     c->st.sgns_tile_ms = ms[2];

@@ -64,7 +64,7 @@ class Stats(C.Structure):
         ("train_kernel_ms", C.c_double), ("eval_kernel_ms", C.c_double),
         ("walk_ms_total", C.c_double), ("train_ms_total", C.c_double), ("eval_ms_total", C.c_double),
         ("sgns_prep_ms", C.c_double), ("sgns_tile_ms", C.c_double), ("graph_ms_total", C.c_double),
-        ("walk_tokens_total", C.c_uint64),
+        ("walk_tokens_total", C.c_uint64), ("train_rows_written", C.c_uint64), ("train_rows_lost", C.c_uint64),
     ]
 
 
@@ -162,6 +162,7 @@ def load_library(path=LIB_PATH):
                                         C.POINTER(C.c_uint32)]),
         "felog_visits_reset": (C.c_int32, [P]),
         "felog_visit_stats": (C.c_int32, [P, P]),
+        "felog_track_lost_updates": (C.c_int32, [P, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -382,6 +383,11 @@ class Felog:
 
     def visits_reset(self):
         self._check(self.lib.felog_visits_reset(self.ctx), "felog_visits_reset")
+
+    def track_lost_updates(self, on=True):
+        """Count the tile trainer's write-backs that replace another tile's
+        update (stats: train_rows_written, train_rows_lost)."""
+        self._check(self.lib.felog_track_lost_updates(self.ctx, int(bool(on))), "felog_track_lost_updates")
 
     def visit_stats(self):
         out = np.zeros(4, np.float64)

@@ -48,7 +48,7 @@ def test_options_struct_layout_matches_header():
     assert felog.Options.seed.offset == 72 and felog.Options.device.offset == 92
     assert felog.Options.update_mode.offset == 96 and felog.Options.compat_walk.offset == 100
     assert ctypes.sizeof(felog.TileShape) == 24 and felog.TileShape.tile_ms.offset == 20
-    assert ctypes.sizeof(felog.Stats) == 8 * 3 + 4 * 2 + 8 * 7 + 8 * 11 + 8
+    assert ctypes.sizeof(felog.Stats) == 8 * 3 + 4 * 2 + 8 * 7 + 8 * 11 + 8 * 3
 
 
 @pytest.mark.parametrize("bad", [dict(size=0), dict(size=600), dict(window=0), dict(window=40),
