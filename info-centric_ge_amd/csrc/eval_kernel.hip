@@ -98,90 +98,90 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
             }
         }
         for (uint64_t m = __ballot(pick); m; m &= m - 1) {
-        const uint32_t src = (uint32_t)__builtin_ctzll(m);
-        const uint32_t v = base + src;
-        const uint64_t b = __shfl(lb, (int)src);
-        const uint32_t deg = (uint32_t)(__shfl(le, (int)src) - b);
-        float score = 0.f;
-        if (deg > 0) {
-            const uint32_t cnt = deg < kEvalNbrs ? deg : kEvalNbrs;
-            // chosen neighbour index held by lane i (i < cnt)
-            uint32_t mine = lane;
-            if (deg > kEvalNbrs) {
-                // Floyd's subset: draw i (lane i, all at once) picks
-                // t_i in [0, j_i], j_i = deg - 30 + i; taken already -> j_i
-                const uint32_t jl = deg - kEvalNbrs + lane;
-                uint32_t t = 0;
-                if (lane < kEvalNbrs) t = draw(seed, v, DOM_EVAL | lane, 0xFFFFFFFFu, round).x % (jl + 1);
-                mine = 0xFFFFFFFFu;
-                for (uint32_t i = 0; i < kEvalNbrs; ++i) {
-                    const uint32_t ti = __shfl(t, (int)i);
-                    const bool dup = __ballot(lane < i && mine == ti) != 0;
-                    if (lane == i) mine = dup ? jl : t;
+            const uint32_t src = (uint32_t)__builtin_ctzll(m);
+            const uint32_t v = base + src;
+            const uint64_t b = __shfl(lb, (int)src);
+            const uint32_t deg = (uint32_t)(__shfl(le, (int)src) - b);
+            float score = 0.f;
+            if (deg > 0) {
+                const uint32_t cnt = deg < kEvalNbrs ? deg : kEvalNbrs;
+                // chosen neighbour index held by lane i (i < cnt)
+                uint32_t mine = lane;
+                if (deg > kEvalNbrs) {
+                    // Floyd's subset: draw i (lane i, all at once) picks
+                    // t_i in [0, j_i], j_i = deg - 30 + i; taken already -> j_i
+                    const uint32_t jl = deg - kEvalNbrs + lane;
+                    uint32_t t = 0;
+                    if (lane < kEvalNbrs) t = draw(seed, v, DOM_EVAL | lane, 0xFFFFFFFFu, round).x % (jl + 1);
+                    mine = 0xFFFFFFFFu;
+                    for (uint32_t i = 0; i < kEvalNbrs; ++i) {
+                        const uint32_t ti = __shfl(t, (int)i);
+                        const bool dup = __ballot(lane < i && mine == ti) != 0;
+                        if (lane == i) mine = dup ? jl : t;
+                    }
                 }
-            }
-            // neighbour vertex of lane i (i < cnt), one gather for all
-            const uint32_t nbr = lane < cnt ? col[b + mine] : v;
-            // Four neighbours at a time, one per 16-lane group: lane l of a
-            // group holds columns 8l .. 8l+7 of every 128-column block and
-            // the group reduces the dot product and the norm over 16 lanes.
-            // All eight rounds of four neighbours are loaded before any is
-            // reduced.
-            const T *rv = syn0 + (uint64_t)v * dp;
-            const T *ru[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t i = 4 * k + grp;
-                ru[k] = syn0 + (uint64_t)__shfl(nbr, (int)(i & 63)) * dp;
-            }
-            float nn = 0.f, dot[8], mm[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) dot[k] = mm[k] = 0.f;
-            for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
-                float x[8], y[8][8];
-                load8(rv, e0, d, x);
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (4 * k < cnt) load8(ru[k], e0, d, y[k]);
-#pragma unroll
-                for (int j = 0; j < 8; ++j) nn += x[j] * x[j];
-#pragma unroll
+                // neighbour vertex of lane i (i < cnt), one gather for all
+                const uint32_t nbr = lane < cnt ? col[b + mine] : v;
+                // Four neighbours at a time, one per 16-lane group: lane l of a
+                // group holds columns 8l .. 8l+7 of every 128-column block and
+                // the group reduces the dot product and the norm over 16 lanes.
+                // All eight rounds of four neighbours are loaded before any is
+                // reduced.
+                const T *rv = syn0 + (uint64_t)v * dp;
+                const T *ru[8];
+    #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    if (4 * k >= cnt) continue;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        dot[k] += x[j] * y[k][j];
-                        mm[k] += y[k][j] * y[k][j];
+                    const uint32_t i = 4 * k + grp;
+                    ru[k] = syn0 + (uint64_t)__shfl(nbr, (int)(i & 63)) * dp;
+                }
+                float nn = 0.f, dot[8], mm[8];
+    #pragma unroll
+                for (int k = 0; k < 8; ++k) dot[k] = mm[k] = 0.f;
+                for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
+                    float x[8], y[8][8];
+                    load8(rv, e0, d, x);
+    #pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        if (4 * k < cnt) load8(ru[k], e0, d, y[k]);
+    #pragma unroll
+                    for (int j = 0; j < 8; ++j) nn += x[j] * x[j];
+    #pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        if (4 * k >= cnt) continue;
+    #pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            dot[k] += x[j] * y[k][j];
+                            mm[k] += y[k][j] * y[k][j];
+                        }
+                    }
+                }
+                nn = group_sum16(nn);
+                const float nv = sqrtf(nn);
+                float sum = 0.f;  // this group's neighbours
+    #pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (4 * k >= cnt) break;
+                    const float dk = group_sum16(dot[k]);
+                    const float nu = sqrtf(group_sum16(mm[k]));
+                    if (4 * k + grp < cnt) sum += (nv == 0.f || nu == 0.f) ? 0.f : dk / (nv * nu);
+                }
+                sum += __shfl_xor(sum, 16);
+                sum += __shfl_xor(sum, 32);
+                score = sum / (float)cnt;
+            }
+            if (lane == 0) {
+                if (recheck) {
+                    if (!(score > kConverge)) budget_cnt[v] = (uint16_t)(round + 1);
+                } else {
+                    scores[v] = score;
+                    if (score > kConverge) {
+                        flags[v] = 1;
+                        ++n_conv;
+                    } else {
+                        ++n_eval;
                     }
                 }
             }
-            nn = group_sum16(nn);
-            const float nv = sqrtf(nn);
-            float sum = 0.f;  // this group's neighbours
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (4 * k >= cnt) break;
-                const float dk = group_sum16(dot[k]);
-                const float nu = sqrtf(group_sum16(mm[k]));
-                if (4 * k + grp < cnt) sum += (nv == 0.f || nu == 0.f) ? 0.f : dk / (nv * nu);
-            }
-            sum += __shfl_xor(sum, 16);
-            sum += __shfl_xor(sum, 32);
-            score = sum / (float)cnt;
-        }
-        if (lane == 0) {
-            if (recheck) {
-                if (!(score > kConverge)) budget_cnt[v] = (uint16_t)(round + 1);
-            } else {
-                scores[v] = score;
-                if (score > kConverge) {
-                    flags[v] = 1;
-                    ++n_conv;
-                } else {
-                    ++n_eval;
-                }
-            }
-        }
         }
     }
     if (lane == 0) {
