@@ -57,37 +57,48 @@ __device__ __forceinline__ float wave_max(float x) {
 //   6 entropy of the transition distribution w / sum w over ln deg (0 if
 //     deg <= 1)
 //   7 1 if v is in the vocabulary (its tokens are trained)
+// Four vertices per wavefront, 16 lanes each (most rows are short); the
+// entropy in one pass as ln S - (sum w ln w) / S.  Rows longer than
+// kHubChunk are left to k_feat_chunk / k_feat_hub.
+__device__ __forceinline__ float group_sum16(float x) {
+    for (int o = 8; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+__device__ __forceinline__ float group_max16(float x) {
+    for (int o = 8; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+    return x;
+}
+
 __global__ __launch_bounds__(256) void k_features(const VertexRec *__restrict__ vrec,
                                                   const uint32_t *__restrict__ col, const float *__restrict__ w,
                                                   const uint8_t *__restrict__ in_vocab, uint32_t V,
                                                   float *__restrict__ out) {
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63, grp = lane >> 4, l = lane & 15;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-    for (uint64_t v = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < V; v += nw) {
-        const VertexRec vr = vrec[v];
-        if (vr.deg > kHubChunk) continue;  // k_feat_chunk / k_feat_hub
-        float ws = 0.f, wm = 0.f, nd = 0.f, ndm = 0.f;
-        for (uint32_t j = lane; j < vr.deg; j += 64) {
-            const float x = w[vr.begin + j];
-            const float du = (float)vrec[col[vr.begin + j]].deg;
-            ws += x;
-            wm = fmaxf(wm, x);
-            nd += du;
-            ndm = fmaxf(ndm, du);
-        }
-        ws = wave_sum(ws);
-        wm = wave_max(wm);
-        nd = wave_sum(nd);
-        ndm = wave_max(ndm);
-        float h = 0.f;
-        if (vr.deg > 1 && ws > 0.f) {
-            for (uint32_t j = lane; j < vr.deg; j += 64) {
-                const float p = w[vr.begin + j] / ws;
-                if (p > 0.f) h -= p * __logf(p);
+    for (uint64_t base = (((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 4; base < V; base += nw * 4) {
+        const uint64_t v = base + grp;
+        VertexRec vr{0, 0};
+        if (v < V) vr = vrec[v];
+        const bool mine = v < V && vr.deg <= kHubChunk;
+        float ws = 0.f, wm = 0.f, nd = 0.f, ndm = 0.f, wl = 0.f;
+        if (mine) {
+            for (uint32_t j = l; j < vr.deg; j += 16) {
+                const float x = w[vr.begin + j];
+                const float du = (float)vrec[col[vr.begin + j]].deg;
+                ws += x;
+                wm = fmaxf(wm, x);
+                nd += du;
+                ndm = fmaxf(ndm, du);
+                if (x > 0.f) wl += x * __logf(x);
             }
-            h = wave_sum(h) / __logf((float)vr.deg);
         }
-        if (lane == 0) {
+        ws = group_sum16(ws);
+        wm = group_max16(wm);
+        nd = group_sum16(nd);
+        ndm = group_max16(ndm);
+        wl = group_sum16(wl);
+        if (mine && l == 0) {
+            const float h = (vr.deg > 1 && ws > 0.f) ? (__logf(ws) - wl / ws) / __logf((float)vr.deg) : 0.f;
             float *f = out + v * kF;
             f[0] = log1pf((float)vr.deg);
             f[1] = log1pf(ws);
@@ -333,9 +344,14 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F
         }
         float y0 = sp[oB3], y1 = sp[oB3 + 1];
         for (int j = 0; j < kH; ++j) {
-            float z = sp[oB2 + j];
-            for (int i = 0; i < kH; ++i) z += sp[oW2 + j * kH + i] * h1[i];
-            const float h = fmaxf(z, 0.f);
+            // four independent partial sums: a 16-deep dependency chain per
+            // output instead of 64
+            float z[4] = {sp[oB2 + j], 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < kH; i += 4)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) z[q] += sp[oW2 + j * kH + i + q] * h1[i + q];
+            const float h = fmaxf((z[0] + z[1]) + (z[2] + z[3]), 0.f);
             y0 += sp[oW3 + j] * h;
             y1 += sp[oW3 + kH + j] * h;
         }
