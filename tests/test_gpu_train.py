@@ -359,10 +359,10 @@ def test_link_prediction_auc_rmat_1m(lp_rmat_1m, base):
 
 def test_bf16_auc_within_fp32(lp_rmat_1m):
     """bf16 tables with stochastic rounding reach the f32 AUC within 0.005
-    (1.1M R-MAT, d = 128, two rounds)."""
+    either side (1.1M R-MAT, d = 128, two rounds)."""
     n, ts, td, tw, pos, neg = lp_rmat_1m
     g32, _, _ = _train_gpu(lp_rmat_1m, 2, size=128)
     g16, _, _ = _train_gpu(lp_rmat_1m, 2, size=128, precision=felog.BF16)
     a32, a16 = _auc(g32, pos, neg), _auc(g16, pos, neg)
     print(f"AUC f32 {a32:.4f} bf16 {a16:.4f}")
-    assert a16 >= a32 - 0.005
+    assert abs(a16 - a32) <= 0.005
