@@ -44,9 +44,37 @@ __device__ __forceinline__ uint32_t first_lane_of(uint32_t t, bool valid) {
     return first;
 }
 
+// The same through a 64-slot hash table in LDS (tab: 128 words of this
+// wavefront's): every valid lane writes its value into slot h(value); the
+// lanes whose value holds the slot take the slot's minimum lane id.  All
+// lanes of one value share a slot, so a value that lost its slot to another
+// loses it on every lane; only those lanes run the ballot loop.
+__device__ __forceinline__ uint32_t first_lane_hashed(uint32_t t, bool valid, uint32_t *tab) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t *key = tab, *low = tab + 64;
+    key[lane] = 0xFFFFFFFFu;
+    low[lane] = 64u;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t h = (t * 0x9E3779B1u) >> 26;
+    if (valid) key[h] = t;
+    __builtin_amdgcn_wave_barrier();
+    const bool held = valid && key[h] == t;
+    if (held) atomicMin(&low[h], lane);
+    __builtin_amdgcn_wave_barrier();
+    uint32_t first = held ? low[h] : lane;
+    const bool rest = valid && !held;
+    if (__builtin_amdgcn_ballot_w64(rest)) {
+        const uint32_t f = first_lane_of(t, rest);
+        if (rest) first = f;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return first;
+}
+
 __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     __shared__ uint32_t s_tok[kPrepWaves][MAXKEEP + 8];
     __shared__ uint32_t s_neg[kPrepWaves][OMAX];
+    __shared__ uint32_t s_hash[kPrepWaves][128];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t *stok = s_tok[wv];
     uint32_t *sneg = s_neg[wv];
@@ -186,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             {
                 const bool valid = lane < npos;
                 const uint32_t t = valid ? stok[lo + lane] : 0xFFFFFFFFu;
-                const uint32_t first = first_lane_of(t, valid);
+                const uint32_t first = first_lane_hashed(t, valid, s_hash[wv]);
                 first_in = first;
                 const bool isf = valid && first == lane;
                 const uint64_t bal = __ballot(isf);
