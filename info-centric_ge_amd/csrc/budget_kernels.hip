@@ -310,9 +310,12 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F
                                                      const felog_budget_model *__restrict__ M, uint32_t max_len,
                                                      uint32_t max_count, uint16_t *__restrict__ cnt,
                                                      uint16_t *__restrict__ len, uint32_t *__restrict__ sum) {
-    __shared__ float sp[FELOG_BUDGET_PARAMS];
+    // The weights are read with wave-uniform indices straight from the
+    // model (scalar loads into SGPR operands of the FMAs): staged in LDS,
+    // every FMA's broadcast read returned a kilobyte per wavefront and the
+    // kernel was bound by that.
+    const float *__restrict__ sp = M->params;
     __shared__ float st[6 * kF + 2 * kO];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)FELOG_BUDGET_PARAMS; i += blockDim.x) sp[i] = M->params[i];
     if (threadIdx.x < (uint32_t)kF) {
         st[threadIdx.x] = M->feat_mean[threadIdx.x];
         st[kF + threadIdx.x] = M->feat_std[threadIdx.x];
