@@ -28,34 +28,39 @@ __device__ __forceinline__ float ld<float>(const float *p) { return *p; }
 template <>
 __device__ __forceinline__ float ld<uint16_t>(const uint16_t *p) { return bf16_to_f32(*p); }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t lane) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // sum over the 16 lanes of this lane's group
 __device__ __forceinline__ float group_sum16(float x) {
     for (int o = 8; o > 0; o >>= 1) x += __shfl_xor(x, o);
     return x;
 }
 
-// Columns e0 .. e0+7 of a row (0 past d); row pitches are multiples of 32
-// elements and e0 of 8, so whole groups are one or two aligned 16-byte loads.
+// Columns e0 .. e0+7 of a row, 0 from column d on.  Row pitches are
+// multiples of 32 elements, e0 one of 8 below d, so the group is always one
+// aligned 16- or 32-byte load inside the row.
 template <typename T>
 __device__ __forceinline__ void load8(const T *row, uint32_t e0, uint32_t d, float (&o)[8]) {
-    if (e0 + 8 <= d) {
-        if constexpr (sizeof(T) == 4) {
-            const float4 a = *(const float4 *)(row + e0), b = *(const float4 *)(row + e0 + 4);
-            o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-            o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
-        } else {
-            const uint4 a = *(const uint4 *)(row + e0);
-            const uint32_t w[4] = {a.x, a.y, a.z, a.w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                o[2 * j] = bf16_to_f32((uint16_t)(w[j] & 0xFFFFu));
-                o[2 * j + 1] = bf16_to_f32((uint16_t)(w[j] >> 16));
-            }
-        }
+    if constexpr (sizeof(T) == 4) {
+        const float4 a = *(const float4 *)(row + e0), b = *(const float4 *)(row + e0 + 4);
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+        o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
     } else {
+        const uint4 a = *(const uint4 *)(row + e0);
+        const uint32_t w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) o[j] = e0 + j < d ? ld(row + e0 + j) : 0.f;
+        for (int j = 0; j < 4; ++j) {
+            o[2 * j] = bf16_to_f32((uint16_t)(w[j] & 0xFFFFu));
+            o[2 * j + 1] = bf16_to_f32((uint16_t)(w[j] >> 16));
+        }
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (e0 + j >= d) o[j] = 0.f;
 }
 
 template <typename T>
@@ -100,8 +105,10 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
         for (uint64_t m = __ballot(pick); m; m &= m - 1) {
             const uint32_t src = (uint32_t)__builtin_ctzll(m);
             const uint32_t v = base + src;
-            const uint64_t b = __shfl(lb, (int)src);
-            const uint32_t deg = (uint32_t)(__shfl(le, (int)src) - b);
+            // read from lane src into scalar registers: everything that
+            // follows on v's degree is wave-uniform control flow
+            const uint64_t b = readlane64(lb, src);
+            const uint32_t deg = (uint32_t)(readlane64(le, src) - b);
             float score = 0.f;
             if (deg > 0) {
                 const uint32_t cnt = deg < kEvalNbrs ? deg : kEvalNbrs;
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                     if (lane < kEvalNbrs) t = draw(seed, v, DOM_EVAL | lane, 0xFFFFFFFFu, round).x % (jl + 1);
                     mine = 0xFFFFFFFFu;
                     for (uint32_t i = 0; i < kEvalNbrs; ++i) {
-                        const uint32_t ti = __shfl(t, (int)i);
+                        const uint32_t ti = __builtin_amdgcn_readlane(t, i);
                         const bool dup = __ballot(lane < i && mine == ti) != 0;
                         if (lane == i) mine = dup ? jl : t;
                     }
@@ -125,33 +132,34 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
                 // Four neighbours at a time, one per 16-lane group: lane l of a
                 // group holds columns 8l .. 8l+7 of every 128-column block and
                 // the group reduces the dot product and the norm over 16 lanes.
-                // All eight rounds of four neighbours are loaded before any is
-                // reduced.
+                // Neighbours past cnt point at v's own row (nbr = v there),
+                // so the loads of a round of four need no test; four rounds
+                // (sixteen neighbours) are loaded before any is reduced.
                 const T *rv = syn0 + (uint64_t)v * dp;
-                const T *ru[8];
+                uint32_t ru[8];  // neighbour of round k in this group
     #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const uint32_t i = 4 * k + grp;
-                    ru[k] = syn0 + (uint64_t)__shfl(nbr, (int)(i & 63)) * dp;
-                }
+                for (int k = 0; k < 8; ++k) ru[k] = __shfl(nbr, 4 * k + (int)grp);
                 float nn = 0.f, dot[8], mm[8];
     #pragma unroll
                 for (int k = 0; k < 8; ++k) dot[k] = mm[k] = 0.f;
                 for (uint32_t e0 = 8 * l; e0 < d; e0 += 128) {
-                    float x[8], y[8][8];
+                    float x[8];
                     load8(rv, e0, d, x);
-    #pragma unroll
-                    for (int k = 0; k < 8; ++k)
-                        if (4 * k < cnt) load8(ru[k], e0, d, y[k]);
     #pragma unroll
                     for (int j = 0; j < 8; ++j) nn += x[j] * x[j];
     #pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        if (4 * k >= cnt) continue;
+                    for (int h = 0; h < 8; h += 4) {
+                        if (4 * h >= cnt) break;
+                        float y[4][8];
     #pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            dot[k] += x[j] * y[k][j];
-                            mm[k] += y[k][j] * y[k][j];
+                        for (int k = 0; k < 4; ++k) load8(syn0 + (uint64_t)ru[h + k] * dp, e0, d, y[k]);
+    #pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+    #pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                dot[h + k] += x[j] * y[k][j];
+                                mm[h + k] += y[k][j] * y[k][j];
+                            }
                         }
                     }
                 }
