@@ -21,13 +21,6 @@ namespace {
 constexpr uint32_t kEvalNbrs = 30;
 constexpr float kConverge = 0.65f;
 
-template <typename T>
-__device__ __forceinline__ float ld(const T *p);
-template <>
-__device__ __forceinline__ float ld<float>(const float *p) { return *p; }
-template <>
-__device__ __forceinline__ float ld<uint16_t>(const uint16_t *p) { return bf16_to_f32(*p); }
-
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t lane) {
     const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, lane);
     const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), lane);
@@ -192,7 +185,16 @@ __global__ __launch_bounds__(256) void k_eval(const uint64_t *__restrict__ row_p
             }
         }
     }
+    // one atomic per workgroup and counter (atomics on one address serialise)
+    __shared__ unsigned long long part[2][4];
     if (lane == 0) {
+        part[0][threadIdx.x >> 6] = n_eval;
+        part[1][threadIdx.x >> 6] = n_conv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        n_eval = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        n_conv = part[1][0] + part[1][1] + part[1][2] + part[1][3];
         if (n_eval) atomicAdd(&counters[C_EVALNUM], n_eval);
         if (n_conv) atomicAdd(&counters[C_EVALCONV], n_conv);
     }

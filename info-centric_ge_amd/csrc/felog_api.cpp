@@ -77,6 +77,8 @@ void free_all(felog_ctx *c) {
                     c->d_w2v_ftok, c->d_w2v_kcnt, c->d_w2v_blk, c->d_w2v_table, c->d_w2v_exp, c->d_scol,
                     c->d_cn, c->hubs.hub, c->hubs.cstart, c->d_feat_part, c->d_budget_sum, c->d_row_ver};
     for (void *p : ptrs) dfree(p);
+    for (auto &b : c->spare) dfree(b.first);
+    c->spare.clear();
     drop_round_graph(c);
     dfree(c->d_rs);
     if (c->h_rs) (void)hipHostFree(c->h_rs);
@@ -272,30 +274,33 @@ felog_status felog_build(felog_ctx *c) {
     if (be == hipSuccess) be = build_vocab_and_tables(c, sc);
     if (be == hipSuccess) be = build_alias(c, sc, edge_src);
     if (be == hipSuccess) be = init_tables(c, &sc);
-    // the sorted sources were either in the scratch or the edge input
-    dfree(c->d_esrc);
+    // the sorted sources were either in the scratch or the edge input's
+    // source block: both blocks are kept for the buffers that follow
+    dkeep(c, c->d_esrc, c->E * 4 + 4);
     c->d_esrc = nullptr;
-    sc.done();
+    sc.done(c);
     TRY(c, be);
     const uint32_t V = c->V;
-    TRY(c, hipMalloc(&c->d_flags, V));
-    TRY(c, hipMemset(c->d_flags, 0, V));
-    TRY(c, hipMalloc(&c->d_scores, (uint64_t)V * 4));
-    TRY(c, hipMemset(c->d_scores, 0, (uint64_t)V * 4));
+    TRY(c, dalloc(c, &c->d_flags, V));
+    TRY(c, hipMemsetAsync(c->d_flags, 0, V, c->stream));
+    TRY(c, dalloc(c, &c->d_scores, (uint64_t)V * 4));
+    TRY(c, hipMemsetAsync(c->d_scores, 0, (uint64_t)V * 4, c->stream));
     // walk buffers
     c->walkers = c->opt.walkers ? c->opt.walkers : V;
     uint32_t slot = (c->opt.min_L + 2 + 3) / 4 * 4;
     if (slot > c->opt.max_walk_len) slot = c->opt.max_walk_len;
     c->slot = slot;
     const uint64_t W = c->walkers;
-    TRY(c, hipMalloc(&c->d_active, W * 4));
-    TRY(c, hipMalloc(&c->d_sel, W * 4));
-    TRY(c, hipMalloc(&c->d_slots, W * slot * 4));
-    TRY(c, hipMalloc(&c->d_lens, W * 4));
-    TRY(c, hipMalloc(&c->d_ovf_base, W * 8));
+    TRY(c, dalloc(c, &c->d_slots, W * slot * 4));
+    TRY(c, dalloc(c, &c->d_active, W * 4));
+    TRY(c, dalloc(c, &c->d_sel, W * 4));
+    TRY(c, dalloc(c, &c->d_lens, W * 4));
+    TRY(c, dalloc(c, &c->d_ovf_base, W * 8));
     c->ovf_cap = c->opt.max_walk_len > slot ? c->opt.overflow_capacity : 0;
-    TRY(c, hipMalloc(&c->d_ovf, (c->ovf_cap ? c->ovf_cap : 1) * 4));
+    TRY(c, dalloc(c, &c->d_ovf, (c->ovf_cap ? c->ovf_cap : 1) * 4));
     TRY(c, round_buffers(c));
+    // the build is done when its last copy and clear are
+    TRY(c, hipStreamSynchronize(c->stream));
     c->last_eva_num = (uint32_t)c->vocab_size;
     c->n_unconverged = (uint32_t)c->vocab_size;
     c->st.build_ms = now_ms() - t0;

@@ -92,9 +92,14 @@ struct felog_ctx {
     uint64_t vocab_size = 0, train_words = 0;
     uint32_t slot = 0;      // per-walker fixed walk slot (vertices)
 
-    // graph input (directed edge list before the CSR is built)
+    // graph input (directed edge list before the CSR is built; the CSR's
+    // columns and weights take over d_edst's and d_ew's blocks)
     uint32_t *d_esrc = nullptr, *d_edst = nullptr;
     float *d_ew = nullptr;
+    // whole device blocks the build has finished with, kept for the round
+    // buffers and tile descriptors that follow instead of being freed and
+    // allocated again: {base, bytes}; freed with the context
+    std::vector<std::pair<void *, size_t>> spare;
     bool have_edges = false, built = false;
 
     // CSR + alias
@@ -114,6 +119,7 @@ struct felog_ctx {
     uint2 *d_uni = nullptr;              // V: {threshold, alias} of the unigram^0.75 sampler
     uint32_t *d_keep_thr = nullptr;      // V: subsampling keep threshold, 0 outside the vocabulary
     double *d_T = nullptr, *d_dT = nullptr;  // n log2 n table and its differences
+    std::vector<double> h_T;             // their host copy (T, then dT)
 
     // embedding tables (f32 or bf16 rows of dp elements, indexed by vertex
     // id); one allocation owned by d_syn0, d_syn1 = d_syn0 + V rows
@@ -243,7 +249,8 @@ struct Scratch {
     }
     size_t mark() const { return top; }
     void release(size_t m) { top = m; }
-    void done();
+    // free everything; or (c set) keep the main block in c's spare blocks
+    void done(felog_ctx *c = nullptr);
 };
 
 // graph_kernels.hip
@@ -351,5 +358,20 @@ hipError_t tile_occupancy(felog_ctx *c, uint32_t rows, uint32_t wpe, int *per_cu
 // Release device memory; a failure here has nothing left to report to.
 inline void dfree(void *p) {
     if (p) (void)hipFree(p);
+}
+// Keep a whole block (base pointer of its allocation, its size) for reuse.
+inline void dkeep(felog_ctx *c, void *p, size_t bytes) {
+    if (p) c->spare.emplace_back(p, bytes);
+}
+// At least `bytes` of device memory: the smallest spare block that holds it
+// and is at most twice its size, else a new allocation.  Either way the
+// block is a whole allocation, released with dfree.
+hipError_t dalloc(felog_ctx *c, void **p, size_t bytes);
+template <typename T>
+hipError_t dalloc(felog_ctx *c, T **p, size_t bytes) {
+    void *q = nullptr;
+    const hipError_t e = dalloc(c, &q, bytes);
+    *p = (T *)q;
+    return e;
 }
 }  // namespace felog

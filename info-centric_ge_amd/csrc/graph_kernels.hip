@@ -216,34 +216,79 @@ __device__ __forceinline__ void wave_argmax(uint64_t &q, uint64_t &j) {
 // Encoded column: light -> its deficit, heavy -> its surplus | kHeavyBit.
 __device__ __forceinline__ uint64_t encode(uint64_t q) { return q < kUnit ? kUnit - q : (q - kUnit) | kHeavyBit; }
 
-// Segments of at most one chunk: one wavefront each, sum, shares and the
-// encoded columns in one pass per step.
+// The butterfly within 16-lane groups.  For a segment of at most 16
+// columns held by lanes 0..15, the 64-lane butterfly's first two steps add
+// +0 (the other lanes hold none of its columns), so this is its sum bit for
+// bit.
+__device__ __forceinline__ double butterfly16(double p) {
+#pragma clang fp contract(off)
+    for (int o = 8; o > 0; o >>= 1) p = p + __shfl_xor(p, o);
+    return p;
+}
+
+// Shares and the encoded columns of the segment [b, b + n) from its sum S:
+// lanes step through the columns (lane l: l, l + step, ...), red() reduces
+// over the lanes that share the segment.
+template <typename W, typename Sum, typename Arg>
+__device__ __forceinline__ void encode_segment(const W *__restrict__ w, uint64_t *__restrict__ val, uint64_t b,
+                                               uint64_t n, double S, uint32_t l, uint32_t step, Sum sum_u64,
+                                               Arg argmax) {
+    const double dn = (double)n;
+    uint64_t sq = 0, qm = 0, jm = ~0ull;
+    if (S > 0.0) {
+        for (uint64_t j = l; j < n; j += step) {
+            const uint64_t q = share(wclamp(w[b + j]), dn, S);
+            sq += q;
+            max_lowest(qm, jm, q, j);
+        }
+        sq = sum_u64(sq);
+        argmax(qm, jm);
+    }
+    const uint64_t rem = n * kUnit - sq;
+    for (uint64_t j = l; j < n; j += step) {
+        uint64_t q = kUnit;
+        if (S > 0.0) q = share(wclamp(w[b + j]), dn, S) + (j == jm ? rem : 0ull);
+        val[b + j] = encode(q);
+    }
+}
+
+// Segments of at most one chunk.  A wavefront takes four consecutive
+// segments: each of at most 16 columns is done by one 16-lane group, the
+// others one after the other by the whole wavefront (sum, shares and the
+// encoded columns, one pass per step).
 template <typename W>
 __global__ __launch_bounds__(256) void k_alias_small(const uint64_t *__restrict__ seg, uint32_t nseg,
                                                      const W *__restrict__ w, uint64_t *__restrict__ val) {
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63, grp = lane >> 4, l16 = lane & 15;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t s = wave; s < nseg; s += nwaves) {
-        const uint64_t b = seg[s], n = seg[s + 1] - b;
-        if (n == 0 || n > kChunk) continue;
-        const double S = butterfly(lane_partial(w + b, n, lane));
-        const double dn = (double)n;
-        uint64_t sq = 0, qm = 0, jm = ~0ull;
-        if (S > 0.0) {
-            for (uint64_t j = lane; j < n; j += 64) {
-                const uint64_t q = share(wclamp(w[b + j]), dn, S);
-                sq += q;
-                max_lowest(qm, jm, q, j);
-            }
-            sq = wave_sum_u64(sq);
-            wave_argmax(qm, jm);
+    auto sum16 = [](uint64_t x) {
+        for (int o = 8; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        return x;
+    };
+    auto argmax16 = [](uint64_t &q, uint64_t &j) {
+        for (int o = 8; o > 0; o >>= 1) max_lowest(q, j, __shfl_xor(q, o), __shfl_xor(j, o));
+    };
+    for (uint32_t s0 = 4 * wave; s0 < nseg; s0 += 4 * nwaves) {
+        const uint32_t s = s0 + grp;
+        uint64_t b = 0, n = 0;
+        if (s < nseg) {
+            b = seg[s];
+            n = seg[s + 1] - b;
         }
-        const uint64_t rem = n * kUnit - sq;
-        for (uint64_t j = lane; j < n; j += 64) {
-            uint64_t q = kUnit;
-            if (S > 0.0) q = share(wclamp(w[b + j]), dn, S) + (j == jm ? rem : 0ull);
-            val[b + j] = encode(q);
+        if (n <= 16) {
+            if (n > 0) {
+                const double S = butterfly16(l16 < n ? wclamp(w[b + l16]) : 0.0);
+                encode_segment(w, val, b, n, S, l16, 16, sum16, argmax16);
+            }
+        }
+        // the larger segments of the four, by the whole wavefront
+        for (uint64_t m = __ballot(l16 == 0 && n > 16 && n <= kChunk); m; m &= m - 1) {
+            const uint32_t g = (uint32_t)__builtin_ctzll(m) >> 4;
+            const uint64_t bg = seg[s0 + g], ng = seg[s0 + g + 1] - bg;
+            const double S = butterfly(lane_partial(w + bg, ng, lane));
+            encode_segment(w, val, bg, ng, S, lane, 64, [](uint64_t x) { return wave_sum_u64(x); },
+                           [](uint64_t &q, uint64_t &j) { wave_argmax(q, j); });
         }
     }
 }
@@ -467,34 +512,47 @@ __global__ void k_vocab_keys(const uint32_t *__restrict__ deg, uint32_t V, uint3
     }
 }
 
-// rank, vocabulary flag (degree >= min-count, or rank 0), the negative
+// rank (a scatter through the sorted order), and per vertex in id order
+// the vocabulary flag (degree >= min-count, or rank 0), the negative
 // sampler's weight deg^0.75 = sqrt(deg) * sqrt(sqrt(deg)) (correctly
-// rounded square roots: the host computes the same bits), and the totals
+// rounded square roots: the host computes the same bits) and the totals
 // (vocabulary size, train words = sum of the vocabulary's degrees).
-__global__ void k_vocab_rank(const uint32_t *__restrict__ order, const uint32_t *__restrict__ deg, uint32_t V,
+__global__ __launch_bounds__(256) void k_vocab_rank(const uint32_t *__restrict__ order, const uint32_t *__restrict__ deg, uint32_t V,
                              uint32_t min_count, uint32_t *__restrict__ rank, uint8_t *__restrict__ inv,
                              double *__restrict__ uw, unsigned long long *totals) {
 #pragma clang fp contract(off)
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t top = order[0];
     unsigned long long vs = 0, tw = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += stride) {
-        const uint32_t v = order[i];
-        const uint32_t d = deg[v];
-        const bool in = d >= min_count || i == 0;
-        rank[v] = i;
-        inv[v] = in ? 1 : 0;
+        rank[order[i]] = i;
+        const uint32_t d = deg[i];
+        const bool in = d >= min_count || i == top;
+        inv[i] = in ? 1 : 0;
         const double x = (double)d;
-        uw[v] = in ? __builtin_sqrt(x) * __builtin_sqrt(__builtin_sqrt(x)) : 0.0;
+        uw[i] = in ? __builtin_sqrt(x) * __builtin_sqrt(__builtin_sqrt(x)) : 0.0;
         vs += in ? 1ull : 0ull;
         tw += in ? d : 0ull;
     }
+    // one pair of atomics per workgroup: adds to one address serialise
     for (int o = 32; o > 0; o >>= 1) {
         vs += __shfl_xor(vs, o);
         tw += __shfl_xor(tw, o);
     }
-    if ((threadIdx.x & 63) == 0 && (vs | tw)) {
-        atomicAdd(&totals[0], vs);
-        atomicAdd(&totals[1], tw);
+    __shared__ unsigned long long part[2][4];
+    const uint32_t wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][wv] = vs;
+        part[1][wv] = tw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        vs = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        tw = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+        if (vs | tw) {
+            atomicAdd(&totals[0], vs);
+            atomicAdd(&totals[1], tw);
+        }
     }
 }
 
@@ -561,12 +619,27 @@ hipError_t Scratch::take(void **p, size_t bytes) {
     return e;
 }
 
-void Scratch::done() {
+void Scratch::done(felog_ctx *c) {
     for (void *p : extra) dfree(p);
     extra.clear();
-    dfree(base);
+    if (c) dkeep(c, base, cap);
+    else dfree(base);
     base = nullptr;
     cap = top = 0;
+}
+
+hipError_t dalloc(felog_ctx *c, void **p, size_t bytes) {
+    size_t best = c->spare.size();
+    for (size_t i = 0; i < c->spare.size(); ++i) {
+        const size_t b = c->spare[i].second;
+        if (b >= bytes && b / 2 <= bytes && (best == c->spare.size() || b < c->spare[best].second)) best = i;
+    }
+    if (best < c->spare.size()) {
+        *p = c->spare[best].first;
+        c->spare.erase(c->spare.begin() + (ptrdiff_t)best);
+        return hipSuccess;
+    }
+    return hipMalloc(p, bytes);
 }
 
 // The largest of the three stages: CSR (second key buffer, two value
@@ -604,9 +677,9 @@ size_t build_scratch_bytes(felog_ctx *c) {
 hipError_t launch_undirect(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w,
                            uint64_t m, bool undirected) {
     const uint64_t E = undirected ? 2 * m : m;
-    HIP_TRY(hipMalloc(&c->d_esrc, E * 4));
-    HIP_TRY(hipMalloc(&c->d_edst, E * 4));
-    HIP_TRY(hipMalloc(&c->d_ew, E * 4));
+    HIP_TRY(hipMalloc(&c->d_esrc, E * 4 + 4));
+    HIP_TRY(hipMalloc(&c->d_edst, E * 4 + 4));
+    HIP_TRY(hipMalloc(&c->d_ew, E * 4 + 4));
     uint32_t *s_src = nullptr, *s_dst = nullptr;
     float *s_w = nullptr;
     HIP_TRY(hipMalloc(&s_src, m * 4));
@@ -727,8 +800,6 @@ hipError_t build_csr(felog_ctx *c, Scratch &sc, uint32_t **edge_src) {
     HIP_TRY(hipMalloc(&c->d_row_ptr, (uint64_t)(V + 1) * 8));
     HIP_TRY(hipMalloc(&c->d_deg, (uint64_t)V * 4));
     HIP_TRY(hipMalloc(&c->d_vrec, (uint64_t)V * sizeof(VertexRec)));
-    HIP_TRY(hipMalloc(&c->d_col, E * 4 + 4));
-    HIP_TRY(hipMalloc(&c->d_w, E * 4 + 4));
 
     // Stable LSD radix sort of (src, dst|w) keeps each vertex's neighbours in
     // input order.  The second key buffer sits at the bottom of the scratch:
@@ -741,6 +812,12 @@ hipError_t build_csr(felog_ctx *c, Scratch &sc, uint32_t **edge_src) {
     HIP_TRY(sc.take(&vals_b, E * 8 + 8));
     k_pack<<<grid_for(E, 256, 65535), 256, 0, c->stream>>>(c->d_edst, c->d_ew, E, vals_a);
     HIP_TRY(hipGetLastError());
+    // the input's destination and weight blocks (E * 4 + 4 bytes each) are
+    // free once packed: they become the CSR's columns and weights
+    c->d_col = c->d_edst;
+    c->d_w = c->d_ew;
+    c->d_edst = nullptr;
+    c->d_ew = nullptr;
     int bits = 1;
     while (bits < 32 && (1ull << bits) < (uint64_t)V) ++bits;
     hipcub::DoubleBuffer<uint32_t> keys(c->d_esrc, keys_b);
@@ -762,14 +839,10 @@ hipError_t build_csr(felog_ctx *c, Scratch &sc, uint32_t **edge_src) {
     // alias build (c->d_esrc stays set when it holds them)
     *edge_src = keys.Current();
     if (keys.Current() != c->d_esrc) {
-        dfree(c->d_esrc);
+        dkeep(c, c->d_esrc, E * 4 + 4);
         c->d_esrc = nullptr;
     }
     sc.release(m0);
-    dfree(c->d_edst);
-    dfree(c->d_ew);
-    c->d_edst = nullptr;
-    c->d_ew = nullptr;
     return hipSuccess;
 }
 
@@ -820,7 +893,7 @@ hipError_t build_vocab_and_tables(felog_ctx *c, Scratch &sc) {
     HIP_TRY(sc.take(&d_tmp, tmp + 16));
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp, keys, ids, (uint64_t)V, 0, 32, st));
     HIP_TRY(hipMemcpyAsync(c->d_vocab_order, ids.Current(), (uint64_t)V * 4, hipMemcpyDeviceToDevice, st));
-    k_vocab_rank<<<grid_for(V, 256, 65535), 256, 0, st>>>(c->d_vocab_order, c->d_deg, V, c->opt.min_count,
+    k_vocab_rank<<<grid_for(V, 256, 1024), 256, 0, st>>>(c->d_vocab_order, c->d_deg, V, c->opt.min_count,
                                                          c->d_vocab_rank, c->d_in_vocab, uw,
                                                          (unsigned long long *)totals);
     HIP_TRY(hipGetLastError());
@@ -836,13 +909,16 @@ hipError_t build_vocab_and_tables(felog_ctx *c, Scratch &sc) {
     sc.release(m0);
     // n log2 n table for the walk statistics
     const uint32_t L = c->opt.max_walk_len;
-    std::vector<double> T(L + 2), dT(L + 1);
+    // (held by the context until the build's last wait: asynchronous copies)
+    std::vector<double> &T = c->h_T;
+    T.assign(2 * L + 3, 0.0);
+    double *dT = T.data() + L + 2;
     for (uint32_t n = 0; n < L + 2; ++n) T[n] = n <= 1 ? 0.0 : (double)n * std::log2((double)n);
     for (uint32_t f = 0; f < L + 1; ++f) dT[f] = T[f + 1] - T[f];
     HIP_TRY(hipMalloc(&c->d_T, (L + 2) * 8));
     HIP_TRY(hipMalloc(&c->d_dT, (L + 1) * 8));
-    HIP_TRY(hipMemcpy(c->d_T, T.data(), (L + 2) * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_dT, dT.data(), (L + 1) * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(c->d_T, T.data(), (L + 2) * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_dT, dT, (L + 1) * 8, hipMemcpyHostToDevice, st));
     return hipSuccess;
 }
 

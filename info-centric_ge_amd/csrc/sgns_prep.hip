@@ -286,8 +286,19 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             __builtin_amdgcn_wave_barrier();
         }
     }
-    // my_words and my_kept are wave-uniform: lane 0 reports them
+    // my_words and my_kept are wave-uniform; one atomic per workgroup and
+    // counter (atomics on one address serialise)
+    __shared__ unsigned long long part[2][kPrepWaves];
     if (lane == 0) {
+        part[0][threadIdx.x >> 6] = my_words;
+        part[1][threadIdx.x >> 6] = my_kept;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kPrepWaves; ++i) {
+            my_words += part[0][i];
+            my_kept += part[1][i];
+        }
         atomicAdd(&a.counters[C_WORDS], my_words);
         atomicAdd(&a.counters[C_KEPT], my_kept);
     }
@@ -322,7 +333,7 @@ hipError_t sgns_reserve(felog_ctx *c, uint64_t n_sent, uint64_t n_tok) {
     c->d_tiles = nullptr;
     c->tiles_cap = 0;
     const uint64_t cap = need + need / 4;
-    HIP_TRY(hipMalloc(&c->d_tiles, cap * kDescBytes));
+    HIP_TRY(dalloc(c, &c->d_tiles, cap * kDescBytes));
     c->tiles_cap = cap;
     return hipSuccess;
 }

@@ -207,7 +207,24 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(WalkArgs a) {
         const uint32_t m2 = __shfl_xor(maxlen, o);
         maxlen = maxlen > m2 ? maxlen : m2;
     }
+    // one atomic per workgroup and counter: atomics on one address
+    // serialise, and a wavefront's each would queue thousands of them at the
+    // end of the launch
+    // (the partials go to the start of each wavefront's own history, which
+    // it no longer reads: more LDS would cost the launch a workgroup per CU)
+    uint32_t *part = hist + (threadIdx.x >> 6) * (kHistLds * 64);
     if (lane == 0) {
+        part[0] = (uint32_t)steps;
+        part[1] = (uint32_t)(steps >> 32);
+        part[2] = maxlen;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kWalkBlock / 64; ++i) {
+            const uint32_t *q = hist + i * (kHistLds * 64);
+            steps += ((unsigned long long)q[1] << 32) | q[0];
+            maxlen = maxlen > q[2] ? maxlen : q[2];
+        }
         atomicAdd(&a.counters[C_STEPS], steps);
         atomicMax(&a.counters[C_MAXLEN], (unsigned long long)maxlen);
     }
@@ -335,21 +352,21 @@ hipError_t round_buffers(felog_ctx *c) {
     const size_t tmp = t1 > t2 ? t1 : t2;
     if (tmp > c->tmp_bytes) {
         dfree(c->d_tmp);
-        HIP_TRY(hipMalloc(&c->d_tmp, tmp));
+        HIP_TRY(dalloc(c, &c->d_tmp, tmp));
         c->tmp_bytes = tmp;
     }
     if ((uint64_t)W + 1 > c->off_cap) {
         dfree(c->d_off);
         dfree(c->d_sent_id);
-        HIP_TRY(hipMalloc(&c->d_off, ((uint64_t)W + 1) * 8));
-        HIP_TRY(hipMalloc(&c->d_sent_id, ((uint64_t)W + 1) * 4));
+        HIP_TRY(dalloc(c, &c->d_off, ((uint64_t)W + 1) * 8));
+        HIP_TRY(dalloc(c, &c->d_sent_id, ((uint64_t)W + 1) * 4));
         c->off_cap = (uint64_t)W + 1;
     }
     // every token of a round sits in a walker's slot or the overflow pool
     const uint64_t tok = (uint64_t)W * c->slot + c->ovf_cap;
     if (tok > c->tok_cap) {
         dfree(c->d_tok);
-        HIP_TRY(hipMalloc(&c->d_tok, tok * 4));
+        HIP_TRY(dalloc(c, &c->d_tok, tok * 4));
         c->tok_cap = tok;
     }
     return hipSuccess;
