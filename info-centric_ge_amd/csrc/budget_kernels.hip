@@ -374,7 +374,12 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const float *__restrict__ F
         n_plain += (uint32_t)__shfl_xor((int)n_plain, o);
         n_plain_vocab += (uint32_t)__shfl_xor((int)n_plain_vocab, o);
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ uint32_t part[3][4];
+    auto add = [](uint32_t p, uint32_t q) { return p + q; };
+    mx = block_combine<4>(mx, part[0], [](uint32_t p, uint32_t q) { return p > q ? p : q; });
+    n_plain = block_combine<4>(n_plain, part[1], add);
+    n_plain_vocab = block_combine<4>(n_plain_vocab, part[2], add);
+    if (threadIdx.x == 0) {
         atomicMax(&sum[0], mx);
         atomicAdd(&sum[1], n_plain);
         atomicAdd(&sum[2], n_plain_vocab);
@@ -443,7 +448,12 @@ __global__ __launch_bounds__(256) void k_visit_stats(const uint32_t *__restrict_
         ent += __shfl_xor(ent, o);
         unv += __shfl_xor(unv, o);
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ double part[3][4];
+    auto add = [](double p, double q) { return p + q; };
+    kl = block_combine<4>(kl, part[0], add);
+    ent = block_combine<4>(ent, part[1], add);
+    unv = block_combine<4>(unv, part[2], add);
+    if (threadIdx.x == 0) {
         atomicAdd(&out[0], kl);
         atomicAdd(&out[1], ent);
         atomicAdd(&out[2], unv);
@@ -498,7 +508,7 @@ hipError_t budget_adam(felog_ctx *c, float *d_p, float *d_grad, float *d_m, floa
 
 hipError_t budget_predict(felog_ctx *c, const float *d_feat, const felog_budget_model *d_model, uint32_t *d_sum) {
     HIP_TRY(hipMemsetAsync(d_sum, 0, 16, c->stream));
-    k_mlp_predict<<<grid_for(c->V, 256, 4096), 256, 0, c->stream>>>(d_feat, c->V, d_model, c->opt.max_walk_len,
+    k_mlp_predict<<<grid_for(c->V, 256, c->cu_count * 4), 256, 0, c->stream>>>(d_feat, c->V, d_model, c->opt.max_walk_len,
                                                                      kBudgetMaxCount, c->d_budget_cnt,
                                                                      c->d_budget_len, d_sum);
     return hipGetLastError();
@@ -531,7 +541,7 @@ hipError_t visit_stats(felog_ctx *c, double *d_out) {
     HIP_TRY(hipMemsetAsync(d_out, 0, 3 * sizeof(double), c->stream));
     const double inv_vis = c->visits_total ? 1.0 / (double)c->visits_total : 0.0;
     const double inv_deg = c->E ? 1.0 / (double)c->E : 0.0;
-    k_visit_stats<<<grid_for(c->V, 256, 4096), 256, 0, c->stream>>>(c->d_visits, c->d_vocab_rank, c->d_deg, c->V,
+    k_visit_stats<<<grid_for(c->V, 256, c->cu_count * 4), 256, 0, c->stream>>>(c->d_visits, c->d_vocab_rank, c->d_deg, c->V,
                                                                      inv_vis, inv_deg,
                                                                      d_out);
     return hipGetLastError();

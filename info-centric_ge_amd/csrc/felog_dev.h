@@ -212,4 +212,18 @@ __device__ __forceinline__ uint32_t hub_of_chunk(const uint64_t *__restrict__ cs
     return lo;
 }
 
+// Combine per-wavefront totals of a workgroup of W wavefronts (x is
+// wave-uniform; part holds W slots of shared memory) into thread 0, which
+// then issues one global atomic per workgroup: atomics on one address
+// serialise at the cache, so one per wavefront of a large grid queues tens
+// of thousands of them behind the kernel's last wave.
+template <int W, typename T, typename Op>
+__device__ __forceinline__ T block_combine(T x, T *part, Op op) {
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < W; ++i) x = op(x, part[i]);
+    return x;
+}
+
 }  // namespace felog
