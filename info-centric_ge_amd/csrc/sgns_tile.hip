@@ -785,7 +785,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         ++my_tiles;
         __syncthreads();  // the next tile reuses the LDS
     }
+    // the pair count goes out once per workgroup (atomics on one address
+    // serialise), combined in the descriptor slots, which no tile reads now
+    for (int o = 32; o > 0; o >>= 1) my_pairs += __shfl_xor(my_pairs, o);
+    unsigned long long *part = (unsigned long long *)(lds + L.o_desc);
+    if (lane == 0) part[wave] = my_pairs;
+    __syncthreads();
     if (tid == 0) {
+        for (uint32_t i = 1; i < kBlock / 64; ++i) my_pairs += part[i];
+        atomicAdd(&a.counters[C_PAIRS], my_pairs);
         atomicAdd(&a.counters[C_GROUPS], my_tiles);
         // every request of this workgroup is complete before it counts as done
         __threadfence();
@@ -794,8 +802,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
             atomicExch(&a.counters[C_DONE], 0ull);
         }
     }
-    for (int o = 32; o > 0; o >>= 1) my_pairs += __shfl_xor(my_pairs, o);
-    if (lane == 0) atomicAdd(&a.counters[C_PAIRS], my_pairs);
     if (TRK && lane == 0) {
         atomicAdd(&a.counters[C_ROWS_WB], (unsigned long long)rows_wb);
         atomicAdd(&a.counters[C_ROWS_LOST], (unsigned long long)rows_lost);
