@@ -402,6 +402,13 @@ def main():
 
     K = args.steps
     walk_gbs = steps * 24 / (walk_ms / 1e3) / 1e9
+    # the walk's access against what the device returns for it: independent
+    # random 32-byte reads over a table the size of the edge records
+    erec_bytes = f.stats()["e_num"] * 32
+    probe_rate = felog.probe_random_reads(erec_bytes, 64_000_000, 32)
+    walk_rate = steps / (walk_ms / 1e3)
+    print(f"[bench] random 32-byte reads over {erec_bytes / 1e9:.2f} GB: {probe_rate / 1e9:.1f}G/s; "
+          f"walk {walk_rate / 1e9:.1f}G steps/s", file=sys.stderr, flush=True)
     # the tile kernel moves the rows; the prep kernel reads the 4-byte tokens
     tile_bytes_word = sgns_bytes_per_word(args.dim, 5, 21, esize) - 4
     sgns_gbs = kept * tile_bytes_word / (tile_ms / 1e3) / 1e9
@@ -445,7 +452,9 @@ def main():
         "eager_tokens_per_step": eager_tok / K,
         "pairs_per_step": pairs / K,
         "walk_roofline": {"bound": "hbm", "achieved": walk_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": walk_gbs / HBM_PEAK_GBS, "bytes_per_step": 24},
+                          "frac": walk_gbs / HBM_PEAK_GBS, "bytes_per_step": 24,
+                          "random_reads_per_s": probe_rate, "random_read_table_bytes": erec_bytes,
+                          "steps_per_random_read": walk_rate / probe_rate},
         "sgns_roofline": {"hbm_GBs": sgns_gbs, "hbm_frac": sgns_gbs / HBM_PEAK_GBS, "mfma_TFs": sgns_tfs,
                           "mfma_peak_TFs": MFMA_PEAK_TFS[args.precision],
                           "mfma_frac": sgns_tfs / MFMA_PEAK_TFS[args.precision],

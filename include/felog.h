@@ -415,6 +415,19 @@ felog_status felog_visit_stats(felog_ctx *ctx, double out[4]);
  * the last round's counts.  on = 0 frees the counters. */
 felog_status felog_track_lost_updates(felog_ctx *ctx, int on);
 
+/* ---- measurement --------------------------------------------------------- */
+/* No counterpart in the reference: a ceiling for the walk's memory access,
+ * measured on the device.  Reads `reads` rows of read_bytes (16, 32, 64 or
+ * 128) at independent pseudo-random rows of a zeroed table of table_bytes
+ * on GPU `device` (four reads in flight per thread, no dependency between
+ * them), five timed launches after a warm-up, and returns the rate in
+ * *reads_per_s.  A walk step is one such 32-byte read whose address the
+ * previous step decides, so the walk's steps per second divided by this
+ * rate (same table size) is the walk's fraction of what the memory system
+ * returns for its access pattern.  Allocates and frees the table itself. */
+felog_status felog_probe_random_reads(int device, uint64_t table_bytes, uint64_t reads,
+                                      uint32_t read_bytes, double *reads_per_s);
+
 /* ---- test hook --------------------------------------------------------- */
 /* Runs the SGNS kernels on a caller-supplied corpus with updates diverted:
  * instead of touching the tables they write, for every tile (a run of

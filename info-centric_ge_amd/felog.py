@@ -163,6 +163,8 @@ def load_library(path=LIB_PATH):
         "felog_visits_reset": (C.c_int32, [P]),
         "felog_visit_stats": (C.c_int32, [P, P]),
         "felog_track_lost_updates": (C.c_int32, [P, C.c_int32]),
+        "felog_probe_random_reads": (C.c_int32, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32,
+                                                 C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -204,6 +206,18 @@ def write_edges(path, src, dst, w=None, binary=True):
     st = fn(path.encode(), _ptr(src), _ptr(dst), _ptr(w), len(src))
     if st != 0:
         raise FelogError(st, "felog_write_edges", path)
+
+
+def probe_random_reads(table_bytes, reads, read_bytes=32, device=0):
+    """Random reads per second the device returns for rows of read_bytes at
+    independent random rows of a table of table_bytes (felog_probe_random_reads):
+    the ceiling the walk's one 32-byte record per step is measured against."""
+    lib = load_library()
+    r = C.c_double()
+    st = lib.felog_probe_random_reads(device, int(table_bytes), int(reads), int(read_bytes), C.byref(r))
+    if st != 0:
+        raise FelogError(st, "felog_probe_random_reads", "")
+    return r.value
 
 
 def default_options():

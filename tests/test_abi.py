@@ -67,6 +67,17 @@ def test_null_context_calls_return_errors():
     assert lib.felog_last_error(None) == b"null context"
 
 
+def test_random_read_probe_rejects_bad_arguments():
+    # argument checks run before any HIP call, so no GPU is needed
+    lib = felog.load_library()
+    r = ctypes.c_double()
+    assert lib.felog_probe_random_reads(0, 1 << 20, 1000, 24, ctypes.byref(r)) == -1
+    assert lib.felog_probe_random_reads(0, 16, 1000, 32, ctypes.byref(r)) == -1
+    assert lib.felog_probe_random_reads(0, 1 << 20, 0, 32, ctypes.byref(r)) == -1
+    assert lib.felog_probe_random_reads(0, 1 << 40, 1000, 16, ctypes.byref(r)) == -1
+    assert lib.felog_probe_random_reads(0, 1 << 20, 1000, 32, None) == -1
+
+
 def test_binary_edge_format_roundtrip(tmp_path):
     s, d, w = graphgen.erdos_renyi(300, 1000, seed=2)
     p = tmp_path / "g.bin"
