@@ -195,7 +195,7 @@ const char *felog_last_error(const felog_ctx *c) {
 felog_status felog_set_edges(felog_ctx *c, const uint32_t *src, const uint32_t *dst, const float *w, uint64_t m,
                              uint32_t v_num, int make_undirected) {
     if (!c) return FELOG_ERR_ARG;
-    if (c->have_edges) return fail(c, FELOG_ERR_STATE, "edges already set on this context");
+    if (c->have_edges || c->d_row_ptr) return fail(c, FELOG_ERR_STATE, "edges already set on this context");
     if (v_num == 0) return fail(c, FELOG_ERR_ARG, "v_num must be > 0");
     if (m > 0 && (!src || !dst || !w)) return fail(c, FELOG_ERR_ARG, "null edge array");
     const uint64_t E = make_undirected ? 2 * m : m;
@@ -279,6 +279,9 @@ felog_status felog_build(felog_ctx *c) {
     dkeep(c, c->d_esrc, c->E * 4 + 4);
     c->d_esrc = nullptr;
     sc.done(c);
+    // the edge input is consumed either way: a failed build cannot be
+    // retried on this context
+    if (be != hipSuccess) c->have_edges = false;
     TRY(c, be);
     const uint32_t V = c->V;
     TRY(c, dalloc(c, &c->d_flags, V));
