@@ -71,6 +71,56 @@ __device__ __forceinline__ uint32_t first_lane_hashed(uint32_t t, bool valid, ui
     return first;
 }
 
+// Pair mode: lanes 0-31 hold one sentence, lanes 32-63 the next.  The
+// lowest lane of this lane's half whose valid value equals this lane's
+// (the lane itself for an invalid lane), as first_lane_of with every
+// ballot cut to the half of the value it looks for.
+__device__ __forceinline__ uint32_t first_lane_of_pair(uint32_t t, bool valid) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t first = lane;
+    const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
+    uint64_t todo = vmask;
+    while (todo) {
+        const uint32_t q = (uint32_t)__builtin_ctzll(todo);
+        const uint32_t tq = __builtin_amdgcn_readlane(t, q);
+        const uint64_t qh = q < 32 ? 0x00000000FFFFFFFFull : 0xFFFFFFFF00000000ull;
+        const uint64_t eq = __builtin_amdgcn_ballot_w64(t == tq) & vmask & qh;
+        if (valid && t == tq && (lane >> 5) == (q >> 5)) first = q;
+        todo &= ~eq;
+    }
+    return first;
+}
+
+// The same through a 32-slot hash table per half in LDS (tab: the
+// wavefront's 128 words, half h using words 64h .. 64h + 63), as
+// first_lane_hashed.
+__device__ __forceinline__ uint32_t first_lane_hashed_pair(uint32_t t, bool valid, uint32_t *tab) {
+    const uint32_t lane = threadIdx.x & 63, hl = lane & 31;
+    uint32_t *key = tab + (lane >> 5) * 64, *low = key + 32;
+    key[hl] = 0xFFFFFFFFu;
+    low[hl] = 64u;
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t h = (t * 0x9E3779B1u) >> 27;
+    if (valid) key[h] = t;
+    __builtin_amdgcn_wave_barrier();
+    const bool held = valid && key[h] == t;
+    if (held) atomicMin(&low[h], lane);
+    __builtin_amdgcn_wave_barrier();
+    uint32_t first = held ? low[h] : lane;
+    const bool rest = valid && !held;
+    if (__builtin_amdgcn_ballot_w64(rest)) {
+        const uint32_t f = first_lane_of_pair(t, rest);
+        if (rest) first = f;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return first;
+}
+
+// kPairs: the pairs of sentences of at most 32 tokens each; else every
+// sentence of the other pairs, one at a time.  Two launches, one of each,
+// cover a round: kept apart, each path gets its own register allocation
+// (about 76 VGPRs each, 100 together).
+template <bool kPairs>
 __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     __shared__ uint32_t s_tok[kPrepWaves][MAXKEEP + 8];
     __shared__ uint32_t s_neg[kPrepWaves][OMAX];
@@ -85,32 +135,10 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     const uint64_t n_sent = a.round_state->n_active;
     const uint32_t round = a.round_state->round;
 
-    // Each wave walks its sentences one ahead: a sentence's offsets and
-    // first 64 tokens are loaded while the one before it is processed, and
-    // its keep thresholds are loaded together with the negatives of its
-    // first groups (so one memory round trip per sentence is exposed).
-    uint64_t sid = (uint64_t)blockIdx.x * kPrepWaves + wv;
-    uint64_t b0 = 0, b0_nx = 0;
-    uint32_t n = 0, n_nx = 0, t_first = 0;
-    if (sid < n_sent) {
-        b0 = a.off[sid];
-        n = (uint32_t)(a.off[sid + 1] - b0);
+    // One sentence of any length on the whole wavefront.
+    auto one = [&](uint64_t sid, uint64_t b0, uint32_t n) {
+        uint32_t t_first = 0;
         if (lane < n) t_first = a.tok[b0 + lane];
-    }
-    if (sid + nwaves < n_sent) {
-        b0_nx = a.off[sid + nwaves];
-        n_nx = (uint32_t)(a.off[sid + nwaves + 1] - b0_nx);
-    }
-    uint64_t b0_nn = 0;
-    uint32_t n_nn = 0, t_nx = 0;
-    auto advance = [&]() {
-        b0 = b0_nx;
-        n = n_nx;
-        t_first = t_nx;
-        b0_nx = b0_nn;
-        n_nx = n_nn;
-    };
-    for (; sid < n_sent; sid += nwaves, advance()) {
         // The sentence's draw for index lane: the subsample coin of raw
         // position lane, the window shrink of kept centre lane and negative
         // lane % k of group lane / k (the first tile's, whose index is the
@@ -122,15 +150,6 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
         uint2 neg_e = make_uint2(0, 0);
         if (lane < (uint32_t)OMAX && lane < k * ((n + B - 1) / B)) neg_e = a.uni[neg_bkt];
         const uint32_t thr_first = lane < n ? a.keep_thr[t_first] : 0u;
-        // the next sentence's first tokens and the offsets of the one after
-        t_nx = 0;
-        if (sid + nwaves < n_sent && lane < n_nx) t_nx = a.tok[b0_nx + lane];
-        b0_nn = 0;
-        n_nn = 0;
-        if (sid + 2 * nwaves < n_sent) {
-            b0_nn = a.off[sid + 2 * nwaves];
-            n_nn = (uint32_t)(a.off[sid + 2 * nwaves + 1] - b0_nn);
-        }
         // ---- filter into stok[0, m) ----
         uint32_t m = 0, words = 0;
         for (uint32_t chunk = 0; chunk < n; chunk += 64) {
@@ -171,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
                 td->nc = 0;
                 td->n_in = 0;
             }
-            continue;
+            return;
         }
         __builtin_amdgcn_wave_barrier();
 
@@ -285,6 +304,218 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             g0 += ng;
             __builtin_amdgcn_wave_barrier();
         }
+    };
+
+    // Two sentences of at most 32 tokens, one per half of the wavefront
+    // (lanes 0-31: sentence s0, lanes 32-63: s0 + 1): the steps of `one`
+    // with lane hl = lane & 31 in place of lane, every ballot cut to the
+    // lane's half and every broadcast taken from the half's own lanes.  Each
+    // lane holds token hl of its half's sentence in t_first.  A half past
+    // the last sentence has no tokens and no slot.
+    auto pair = [&](uint64_t s0, uint32_t nA, uint32_t nB, uint32_t t_first) {
+        const uint32_t hl = lane & 31, hb = lane & 32;
+        const uint64_t hm = hb ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+        const uint64_t lt_half = lt_mask & hm;
+        const uint64_t sid = s0 + (hb >> 5);
+        const bool present = sid < n_sent;
+        const uint32_t n = present ? (hb ? nB : nA) : 0u;
+        uint32_t *ptok = stok + 2 * hb;    // kept tokens: 64 words per half
+        uint32_t *pneg = stok + 128 + hb;  // negatives: 32 words per half
+        const u32x4 r0 = draw(a.seed, hl, DOM_SGNS, (uint32_t)sid, round);
+        const uint32_t neg_bkt = alias_bucket(r0.z, a.V);
+        uint2 neg_e = make_uint2(0, 0);
+        if (hl < k * ((n + B - 1) / B)) neg_e = a.uni[neg_bkt];
+        const uint32_t thr = hl < n ? a.keep_thr[t_first] : 0u;
+        // ---- filter (one chunk) ----
+        const uint32_t t = t_first;
+        int status = 0;  // 0 none, 1 drop, 2 keep, 3 break
+        if (hl < n) {
+            if (a.compat_break && (a.in_vocab[t] == 0 || a.vocab_rank[t] == 0)) status = 3;
+            else if (thr == 0u) status = 1;
+            else status = (thr == 0xFFFFFFFFu || r0.x < thr) ? 2 : 1;
+        }
+        const uint64_t brk_bal = __ballot(status == 3) & hm;
+        const uint32_t brk = brk_bal ? (uint32_t)__builtin_ctzll(brk_bal) - hb : 0xFFFFFFFFu;
+        const bool keep = status == 2 && hl < brk;
+        const uint64_t bal = __ballot(keep) & hm;
+        if (keep) ptok[__popcll(bal & lt_half)] = t;
+        const uint32_t m = (uint32_t)__popcll(bal);
+        const uint32_t words = brk != 0xFFFFFFFFu ? brk + 1 : n;
+        my_words += __builtin_amdgcn_readlane(words, 0) + __builtin_amdgcn_readlane(words, 32);
+        my_kept += __builtin_amdgcn_readlane(m, 0) + __builtin_amdgcn_readlane(m, 32);
+        if (present && m < 2 && hl == 0) {
+            // slot sid stays empty: no centre, nothing to train
+            TileDesc *td = (TileDesc *)(a.tiles + sid * kDescBytes);
+            td->nc = 0;
+            td->n_in = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+
+        // ---- tiles: each half runs its own, finished halves idle ----
+        uint32_t g0 = 0;
+        bool more = present && m >= 2;
+        while (__ballot(more)) {
+            uint32_t ng = 0, c0 = 0, nc = 0, nneg = 0, lo = 0, npos = 0;
+            if (more) {
+                ng = tile_groups(g0, m, B, w, k, a.rmax);
+                c0 = g0 * B;
+                const uint32_t c_end = (g0 + ng) * B < m ? (g0 + ng) * B : m;
+                nc = c_end - c0;
+                nneg = k * ng;
+                lo = c0 >= w ? c0 - w : 0;
+                const uint32_t hi = c_end + w < m ? c_end + w : m;
+                npos = hi - lo;
+            }
+            uint64_t slot = sid;
+            if (more && g0 > 0 && hl == 0) slot = n_sent + atomicAdd(&a.counters[C_TILES], 1ull);
+            slot = __shfl(slot, (int)hb);
+            if (more && hl < nneg) {
+                if (g0 == 0) {
+                    pneg[hl] = r0.w < neg_e.x ? neg_bkt : neg_e.y;
+                } else {
+                    const u32x4 r = draw(a.seed, g0 * k + hl, DOM_SGNS, (uint32_t)sid, round);
+                    const uint32_t bkt = alias_bucket(r.z, a.V);
+                    const uint2 e = a.uni[bkt];
+                    pneg[hl] = r.w < e.x ? bkt : e.y;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // a slot past the buffer is counted; the host reports the overflow
+            const bool live = more && slot < a.tiles_cap;
+            TileDesc *td = (TileDesc *)(a.tiles + slot * kDescBytes);
+            // distinct input rows over positions [lo, lo + npos)
+            uint32_t first_in;
+            {
+                const bool valid = live && hl < npos;
+                const uint32_t ti = valid ? ptok[lo + hl] : 0xFFFFFFFFu;
+                const uint32_t first = first_lane_hashed_pair(ti, valid, s_hash[wv]);
+                first_in = first;
+                const bool isf = valid && first == lane;
+                const uint64_t bi = __ballot(isf) & hm;
+                const uint32_t idx = (uint32_t)__popcll(bi & lt_half);
+                const uint32_t my = __shfl(idx, (int)first);
+                const uint32_t n_in = (uint32_t)__popcll(bi);
+                if (live) {
+                    if (isf) td->rin_tok[idx] = ti;
+                    if (hl >= n_in) td->rin_tok[hl] = 0xFFFFFFFFu;
+                    td->rin_tok[32 + hl] = 0xFFFFFFFFu;
+                    td->pos_rin[hl] = valid ? (uint8_t)my : 0;
+                    td->pos_rin[32 + hl] = 0;
+                    if (hl == 0) {
+                        td->sid = (uint32_t)sid; td->c0 = c0; td->nc = nc; td->g0 = g0;
+                        td->lo = lo; td->npos = npos; td->n_in = n_in;
+                    }
+                }
+            }
+            // distinct output rows over [centres..., negatives...]
+            {
+                const uint32_t nl = nc + nneg;
+                const bool valid = live && hl < nl;
+                uint32_t to = 0xFFFFFFFFu;
+                if (valid) to = hl < nc ? ptok[c0 + hl] : pneg[hl - nc];
+                // a tile over its whole window span: the centres' first lanes
+                // are the input pass's, each negative takes the lowest lane of
+                // its half holding its row; other tiles dedupe from scratch
+                const bool span = c0 == lo && nc == npos;
+                uint32_t first = (span && hl < nc) ? first_in : lane;
+                {
+                    const uint32_t f = first_lane_of_pair(to, valid && !span);
+                    if (valid && !span) first = f;
+                }
+                const uint32_t jA = __builtin_amdgcn_readlane((live && span) ? nneg : 0u, 0);
+                const uint32_t jB = __builtin_amdgcn_readlane((live && span) ? nneg : 0u, 32);
+                for (uint32_t j = 0; j < (jA > jB ? jA : jB); ++j) {
+                    const uint32_t src = hb + ((nc + j) & 31);
+                    const uint32_t tq = __shfl(to, (int)src);
+                    const uint64_t eq = __ballot(valid && to == tq) & hm;
+                    if (span && j < nneg && lane == src) first = (uint32_t)__builtin_ctzll(eq);
+                }
+                const bool isf = valid && first == lane;
+                const uint64_t bo = __ballot(isf) & hm;
+                const uint32_t idx = (uint32_t)__popcll(bo & lt_half);
+                const uint32_t my = __shfl(idx, (int)first);
+                const uint32_t n_out = (uint32_t)__popcll(bo);
+                // output row of negative q sits on lane nc + q of the half
+                const uint32_t my_neg = __shfl(my, (int)(hb + ((nc + hl) & 31)));
+                // negatives of a centre's group equal to the centre (see one)
+                uint32_t coll = 0;
+                {
+                    const uint32_t grp = (uint32_t)((float)hl / (float)B);
+                    const uint32_t src = nc + grp * k;
+                    for (uint32_t j = 0; j < k; ++j) coll += __shfl(to, (int)(hb + ((src + j) & 31))) == to;
+                }
+                if (live) {
+                    if (isf) td->rout_tok[idx] = to;
+                    if (hl >= n_out) td->rout_tok[hl] = 0xFFFFFFFFu;
+                    td->cen_rout[hl] = (valid && hl < nc) ? (uint8_t)my : 0;
+                    td->neg_rout[hl] = hl < nneg ? (uint8_t)my_neg : 0;
+                    // window shrink of kept centre c0 + hl (the first tile's: r0)
+                    uint32_t by = r0.y;
+                    if (c0 != 0) by = draw(a.seed, c0 + hl, DOM_SGNS, (uint32_t)sid, round).y;
+                    td->bshrink[hl] = hl < nc ? (uint8_t)(by % w) : 0;
+                    td->cen_coll[hl] = hl < nc ? (uint8_t)coll : 0;
+                    if (hl == 0) td->n_out = n_out;
+                }
+            }
+            if (more) {
+                g0 += ng;
+                more = g0 * B < m;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
+
+    // Sentences go to wavefronts in pairs (2p, 2p + 1).  A pair of
+    // sentences of at most 32 tokens each is done in one pass, one sentence
+    // per half of the wavefront; otherwise each takes the whole wavefront in
+    // turn.  A pair's offsets and tokens are loaded while the pair before it
+    // is processed.
+    auto offsets = [&](uint64_t p, uint64_t &bA, uint32_t &nA, uint64_t &bB, uint32_t &nB) {
+        bA = bB = 0;
+        nA = nB = 0;
+        if (2 * p < n_sent) {
+            bA = a.off[2 * p];
+            bB = a.off[2 * p + 1];
+            nA = (uint32_t)(bB - bA);
+            if (2 * p + 1 < n_sent) nB = (uint32_t)(a.off[2 * p + 2] - bB);
+        }
+    };
+    // token lane & 31 of the lane's half's sentence, for a pair of short ones
+    auto pair_tokens = [&](uint64_t bA, uint32_t nA, uint64_t bB, uint32_t nB) -> uint32_t {
+        const uint32_t hl = lane & 31;
+        const uint32_t nh = lane < 32 ? nA : nB;
+        if (!kPairs || nA > 32 || nB > 32 || hl >= nh) return 0u;
+        return a.tok[(lane < 32 ? bA : bB) + hl];
+    };
+    uint64_t pr = (uint64_t)blockIdx.x * kPrepWaves + wv;
+    uint64_t bA, bB, bA_nx, bB_nx;
+    uint32_t nA, nB, nA_nx, nB_nx;
+    offsets(pr, bA, nA, bB, nB);
+    uint32_t t_first = pair_tokens(bA, nA, bB, nB);
+    offsets(pr + nwaves, bA_nx, nA_nx, bB_nx, nB_nx);
+    for (; 2 * pr < n_sent; pr += nwaves) {
+        // the next pair's tokens and the offsets of the pair after it
+        const uint32_t t_nx = pair_tokens(bA_nx, nA_nx, bB_nx, nB_nx);
+        uint64_t bA_nn, bB_nn;
+        uint32_t nA_nn, nB_nn;
+        offsets(pr + 2 * nwaves, bA_nn, nA_nn, bB_nn, nB_nn);
+        if (nA <= 32 && nB <= 32) {
+            if constexpr (kPairs) pair(2 * pr, nA, nB, t_first);
+        } else {
+            if constexpr (!kPairs) {
+                one(2 * pr, bA, nA);
+                if (2 * pr + 1 < n_sent) one(2 * pr + 1, bB, nB);
+            }
+        }
+        bA = bA_nx;
+        nA = nA_nx;
+        bB = bB_nx;
+        nB = nB_nx;
+        t_first = t_nx;
+        bA_nx = bA_nn;
+        nA_nx = nA_nn;
+        bB_nx = bB_nn;
+        nB_nx = nB_nn;
     }
     // my_words and my_kept are wave-uniform; one atomic per workgroup and
     // counter (atomics on one address serialise)
@@ -385,7 +616,8 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
     // lost updates are a property of write-back tiles that train the tables
     a.row_ver = (dbg || !a.writeback) ? nullptr : c->d_row_ver;
     // the sentence count is read on the device: a fixed grid strides over it
-    k_sgns_prep<<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
+    k_sgns_prep<true><<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
+    k_sgns_prep<false><<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
     HIP_TRY(hipGetLastError());
     if (mid) HIP_TRY(hipEventRecord(mid, c->stream));
     return launch_tile(c, a);
