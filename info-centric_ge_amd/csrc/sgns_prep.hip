@@ -71,6 +71,12 @@ __device__ __forceinline__ uint32_t first_lane_hashed(uint32_t t, bool valid, ui
     return first;
 }
 
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t lane) {
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Pair mode: lanes 0-31 hold one sentence, lanes 32-63 the next.  The
 // lowest lane of this lane's half whose valid value equals this lane's
 // (the lane itself for an invalid lane), as first_lane_of with every
@@ -487,35 +493,52 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
         if (!kPairs || nA > 32 || nB > 32 || hl >= nh) return 0u;
         return a.tok[(lane < 32 ? bA : bB) + hl];
     };
-    uint64_t pr = (uint64_t)blockIdx.x * kPrepWaves + wv;
-    uint64_t bA, bB, bA_nx, bB_nx;
-    uint32_t nA, nB, nA_nx, nB_nx;
-    offsets(pr, bA, nA, bB, nB);
-    uint32_t t_first = pair_tokens(bA, nA, bB, nB);
-    offsets(pr + nwaves, bA_nx, nA_nx, bB_nx, nB_nx);
-    for (; 2 * pr < n_sent; pr += nwaves) {
-        // the next pair's tokens and the offsets of the pair after it
-        const uint32_t t_nx = pair_tokens(bA_nx, nA_nx, bB_nx, nB_nx);
-        uint64_t bA_nn, bB_nn;
-        uint32_t nA_nn, nB_nn;
-        offsets(pr + 2 * nwaves, bA_nn, nA_nn, bB_nn, nB_nn);
-        if (nA <= 32 && nB <= 32) {
-            if constexpr (kPairs) pair(2 * pr, nA, nB, t_first);
-        } else {
-            if constexpr (!kPairs) {
-                one(2 * pr, bA, nA);
-                if (2 * pr + 1 < n_sent) one(2 * pr + 1, bB, nB);
+    const uint64_t wave_id = (uint64_t)blockIdx.x * kPrepWaves + wv;
+    if constexpr (kPairs) {
+        uint64_t pr = wave_id;
+        uint64_t bA, bB, bA_nx, bB_nx;
+        uint32_t nA, nB, nA_nx, nB_nx;
+        offsets(pr, bA, nA, bB, nB);
+        uint32_t t_first = pair_tokens(bA, nA, bB, nB);
+        offsets(pr + nwaves, bA_nx, nA_nx, bB_nx, nB_nx);
+        for (; 2 * pr < n_sent; pr += nwaves) {
+            // the next pair's tokens and the offsets of the pair after it
+            const uint32_t t_nx = pair_tokens(bA_nx, nA_nx, bB_nx, nB_nx);
+            uint64_t bA_nn, bB_nn;
+            uint32_t nA_nn, nB_nn;
+            offsets(pr + 2 * nwaves, bA_nn, nA_nn, bB_nn, nB_nn);
+            if (nA <= 32 && nB <= 32) pair(2 * pr, nA, nB, t_first);
+            bA = bA_nx;
+            nA = nA_nx;
+            bB = bB_nx;
+            nB = nB_nx;
+            t_first = t_nx;
+            bA_nx = bA_nn;
+            nA_nx = nA_nn;
+            bB_nx = bB_nn;
+            nB_nx = nB_nn;
+        }
+    } else {
+        // the other pairs: 64 pairs tested at a time, lane j pair base + j,
+        // and only those holding a sentence of more than 32 tokens are run
+        for (uint64_t base = wave_id * 64; 2 * base < n_sent; base += nwaves * 64) {
+            const uint64_t p = base + lane;
+            uint64_t o0 = 0, o1 = 0, o2 = 0;
+            bool big = false;
+            if (2 * p < n_sent) {
+                o0 = a.off[2 * p];
+                o1 = a.off[2 * p + 1];
+                o2 = 2 * p + 1 < n_sent ? a.off[2 * p + 2] : o1;
+                big = o1 - o0 > 32 || o2 - o1 > 32;
+            }
+            for (uint64_t bm = __ballot(big); bm; bm &= bm - 1) {
+                const uint32_t j = (uint32_t)__builtin_ctzll(bm);
+                const uint64_t q = base + j;
+                const uint64_t bA = readlane64(o0, j), bB = readlane64(o1, j), eB = readlane64(o2, j);
+                one(2 * q, bA, (uint32_t)(bB - bA));
+                if (2 * q + 1 < n_sent) one(2 * q + 1, bB, (uint32_t)(eB - bB));
             }
         }
-        bA = bA_nx;
-        nA = nA_nx;
-        bB = bB_nx;
-        nB = nB_nx;
-        t_first = t_nx;
-        bA_nx = bA_nn;
-        nA_nx = nA_nn;
-        bB_nx = bB_nn;
-        nB_nx = nB_nn;
     }
     // my_words and my_kept are wave-uniform; one atomic per workgroup and
     // counter (atomics on one address serialise)
