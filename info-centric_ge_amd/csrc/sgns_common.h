@@ -72,6 +72,9 @@ struct Params {
     // update products (a wavefront's item is one row block's columns
     // part, part + parts, ...: dp / parts columns)
     uint32_t win_shift, col_parts;
+    // the longest sentence of the corpus when known on the device (the walk
+    // round's maximum length for the library's own corpus), else null
+    const unsigned long long *max_sent_len;
     unsigned long long *counters;
     // test hook: when dbg_max > 0 the tables are not touched
     uint32_t dbg_max;

@@ -131,6 +131,8 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     __shared__ uint32_t s_tok[kPrepWaves][MAXKEEP + 8];
     __shared__ uint32_t s_neg[kPrepWaves][OMAX];
     __shared__ uint32_t s_hash[kPrepWaves][128];
+    // no sentence over 32 tokens: nothing for the whole-wavefront launch
+    if (!kPairs && a.max_sent_len && *a.max_sent_len <= 32) return;
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint32_t *stok = s_tok[wv];
     uint32_t *sneg = s_neg[wv];
@@ -628,6 +630,9 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
     a.writeback = c->opt.update_mode != FELOG_UPDATE_ATOMIC;
     a.seed = c->opt.seed;
     a.counters = (unsigned long long *)c->d_counters;
+    // the library's corpus buffers hold the last walk round's paths, whose
+    // longest length the walk counted; other corpora are scanned
+    a.max_sent_len = (d_off == c->d_off && d_tok == c->d_tok) ? a.counters + C_MAXLEN : nullptr;
     a.dbg_max = dbg ? dbg->max_groups : 0;
     a.dbg_groups = dbg ? dbg->groups : nullptr;
     a.dbg_rows = dbg ? dbg->rows : nullptr;
