@@ -1,7 +1,8 @@
 // sgns_prep.hip - first half of an SGNS training pass, and the pass driver.
 //
-// k_sgns_prep runs one wavefront per sentence and does the integer work of
-// the skip-gram pass (reference paths relative to the reference repository):
+// k_sgns_prep runs one sentence per wavefront, or two of at most 32 tokens
+// per wavefront (one per half), and does the integer work of the
+// skip-gram pass (reference paths relative to the reference repository):
 //   sentence filtering  src/word2vec.cu:1196-1232  (vocabulary lookup,
 //                       subsampling keep = min(1, (sqrt(c/sT)+1) sT/c) with
 //                       c = degree and the draw Philox(seed; position,
