@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
     auto pair_tokens = [&](uint64_t bA, uint32_t nA, uint64_t bB, uint32_t nB) -> uint32_t {
         const uint32_t hl = lane & 31;
         const uint32_t nh = lane < 32 ? nA : nB;
-        if (!kPairs || nA > 32 || nB > 32 || hl >= nh) return 0u;
+        if (nA > 32 || nB > 32 || hl >= nh) return 0u;
         return a.tok[(lane < 32 ? bA : bB) + hl];
     };
     const uint64_t wave_id = (uint64_t)blockIdx.x * kPrepWaves + wv;
