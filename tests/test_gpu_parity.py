@@ -184,6 +184,39 @@ def test_sgns_long_sentences_vs_fp64(er, rows):
             assert np.linalg.norm(x - y) <= 1e-5 * max(np.linalg.norm(y), 1e-30), (rows, name, i)
 
 
+def test_sgns_paired_sentences_vs_fp64(er):
+    """Sentences of at most 32 tokens are prepared two per wavefront (one per
+    half).  With 21 centres per group, lengths 23-32 give most such
+    sentences a second tile, starting past centre 0 with its own negatives
+    and window shrinks; 2-22 give one tile; an odd count of sentences leaves
+    the last wavefront half empty, and a few sentences of 33-60 tokens send
+    their pair to the whole-wavefront path.  Every tile matches FP64."""
+    f, _ = er
+    rng = np.random.default_rng(23)
+    lens = np.concatenate([rng.integers(23, 33, 40), rng.integers(2, 23, 16), rng.integers(33, 61, 5)])
+    rng.shuffle(lens)
+    assert len(lens) % 2 == 1
+    off_s = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    tok_s = rng.integers(0, 10_000, int(off_s[-1])).astype(np.uint32)
+    s0, s1 = _random_tables(10_000, 128, 11)
+    f.set_embeddings(s0, s1)
+    got = f.debug_sgns(off_s, tok_s, 3, 0.025, 4096)
+    ref = orc.sgns_groups(_tables_for_oracle(f), off_s, tok_s, s0, s1, d=128, window=10, negative=5, batch=0,
+                          alpha=0.025, rnd=3, seed=SEED, mode=0)
+    assert len(got["groups"]) == ref["n_groups"] > len(lens)  # second tiles exist
+    assert sum(1 for g in got["groups"] if int(g[1]) > 0) >= 20  # tiles starting past centre 0
+    key = {(int(g[0]), int(g[1])): i for i, g in enumerate(ref["groups"])}
+    for i, g in enumerate(got["groups"]):
+        j = key[(int(g[0]), int(g[1]))]
+        assert np.array_equal(g, ref["groups"][j]), (i, g, ref["groups"][j])
+        assert np.array_equal(got["rows"][i], ref["rows"][j]), i
+        n_in, n_out = int(g[3]), int(g[4])
+        for name, shape in (("logits", (n_in, n_out)), ("dv", (n_in, 128)), ("du", (n_out, 128))):
+            x = got[name][i][: shape[0], : shape[1]].astype(np.float64)
+            y = ref[name][j][: shape[0], : shape[1]]
+            assert np.linalg.norm(x - y) <= 1e-5 * max(np.linalg.norm(y), 1e-30), (name, i)
+
+
 def test_sgns_groups_bf16_vs_fp64():
     """bf16 tables: X on bf16 MFMA from the exact bf16 rows; the gradient
     products split G into bf16 hi + lo (relative residue below 2^-16), so the
