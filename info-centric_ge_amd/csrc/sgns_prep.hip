@@ -27,6 +27,10 @@ namespace sgns {
 namespace {
 
 constexpr int kPrepWaves = kBlock / 64;
+// Workgroups per CU of the two-sentence launch (6 are resident at 76 VGPRs):
+// at 2.24M sentences a grid of 16 per CU took 1.32 ms, 48 1.25, 96 1.23 and
+// 192 1.42 ms, larger grids paying for each workgroup's closing atomics.
+constexpr int kPairGridPerCu = 96;
 
 // Lowest lane whose valid value equals this lane's (the lane itself for
 // an invalid lane): one uniform pass per distinct value, each a broadcast
@@ -556,8 +560,9 @@ __global__ __launch_bounds__(kBlock) void k_sgns_prep(Params a) {
             my_words += part[0][i];
             my_kept += part[1][i];
         }
-        atomicAdd(&a.counters[C_WORDS], my_words);
-        atomicAdd(&a.counters[C_KEPT], my_kept);
+        // a workgroup with no sentence adds nothing
+        if (my_words) atomicAdd(&a.counters[C_WORDS], my_words);
+        if (my_kept) atomicAdd(&a.counters[C_KEPT], my_kept);
     }
 }
 
@@ -645,7 +650,7 @@ hipError_t sgns_enqueue_into(felog_ctx *c, const uint64_t *d_off, const uint32_t
     // lost updates are a property of write-back tiles that train the tables
     a.row_ver = (dbg || !a.writeback) ? nullptr : c->d_row_ver;
     // the sentence count is read on the device: a fixed grid strides over it
-    k_sgns_prep<true><<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
+    k_sgns_prep<true><<<c->cu_count * kPairGridPerCu, kBlock, 0, c->stream>>>(a);
     k_sgns_prep<false><<<c->cu_count * 16, kBlock, 0, c->stream>>>(a);
     HIP_TRY(hipGetLastError());
     if (mid) HIP_TRY(hipEventRecord(mid, c->stream));
