@@ -29,7 +29,7 @@ namespace {
 constexpr int kPrepWaves = kBlock / 64;
 // Workgroups per CU of the two-sentence launch (6 are resident at 76 VGPRs):
 // at 2.24M sentences a grid of 16 per CU took 1.32 ms, 48 1.25, 96 1.23 and
-// 192 1.42 ms, larger grids paying for each workgroup's closing atomics.
+// 192 1.42 ms; above that the time grows with the workgroup count.
 constexpr int kPairGridPerCu = 96;
 
 // Lowest lane whose valid value equals this lane's (the lane itself for
